@@ -1,0 +1,325 @@
+"""ctypes binding of the custen C ABI (include/cuSten.h).
+
+The functions here keep the reference's names and argument order
+(cuSten/src/struct/cuSten_struct_functions.h, cuSten/src/kernels/stencil_kernels.h),
+so a Python caller writes the same sequence a C caller of the reference does:
+
+    h = cuSten_t()
+    cuStenCreate2DXYp(h, 0, numTiles, nx, ny, BX, BY, out, inp, weights, 5, 2, 2, 5, 2, 2)
+    cuStenCompute2DXYp(h, DEVICE)
+    cuStenDestroy2DXYp(h)
+
+Data arguments may be torch tensors (device or host), numpy arrays (host),
+raw integer addresses or None.  Where the C library records an error, the
+wrapper raises CustenError (the C entry point `checkError` keeps the
+reference's print-and-exit behaviour for C callers).
+
+The native library must be built (custen_amd.build); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+DEVICE = 0
+HOST = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcusten.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cuSten.h")
+
+c_int = ctypes.c_int
+c_bool = ctypes.c_bool
+c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+c_uint64 = ctypes.c_uint64
+
+
+class CustenError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"custen error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class cuSten_t(ctypes.Structure):
+    """Mirror of cuSten_t (include/cuSten.h; reference layout
+    cuSten/src/struct/cuSten_struct_type.h:84-122 plus the trailing plan)."""
+
+    _fields_ = [
+        ("deviceNum", c_int),
+        ("numStreams", c_int),
+        ("numTiles", c_int),
+        ("nx", c_int),
+        ("ny", c_int),
+        ("nyTile", c_int),
+        ("numSten", c_int),
+        ("numStenLeft", c_int),
+        ("numStenRight", c_int),
+        ("numStenTop", c_int),
+        ("numStenBottom", c_int),
+        ("numStenHoriz", c_int),
+        ("numStenVert", c_int),
+        ("BLOCK_X", c_int),
+        ("BLOCK_Y", c_int),
+        ("xGrid", c_int),
+        ("yGrid", c_int),
+        ("mem_shared", c_int),
+        ("dataInput", ctypes.POINTER(c_void_p)),
+        ("dataOutput", ctypes.POINTER(c_void_p)),
+        ("uVel", ctypes.POINTER(c_void_p)),
+        ("vVel", ctypes.POINTER(c_void_p)),
+        ("weights", c_void_p),
+        ("coe", c_void_p),
+        ("coeDx", c_double),
+        ("coeDy", c_double),
+        ("numCoe", c_int),
+        ("nxLocal", c_int),
+        ("nyLocal", c_int),
+        ("boundaryTop", ctypes.POINTER(c_void_p)),
+        ("boundaryBottom", ctypes.POINTER(c_void_p)),
+        ("numBoundaryTop", c_int),
+        ("numBoundaryBottom", c_int),
+        ("streams", ctypes.POINTER(c_void_p)),
+        ("events", ctypes.POINTER(c_void_p)),
+        ("devFunc", c_void_p),
+        ("plan", c_void_p),
+    ]
+
+
+class custenCHParams(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int),
+        ("lx", c_double),
+        ("D", c_double),
+        ("gamma", c_double),
+        ("dtOverDx", c_double),
+    ]
+
+
+_H = ctypes.POINTER(cuSten_t)
+_P = c_void_p
+
+# name -> (restype, argtypes)
+_X_CREATE = [_H, c_int, c_int, c_int, c_int, c_int, c_int, _P, _P, _P, c_int, c_int, c_int]
+_XFUN_CREATE = _X_CREATE + [c_int, _P]
+_YNPFUN_CREATE = _X_CREATE + [_P]
+_XY_CREATE = [_H, c_int, c_int, c_int, c_int, c_int, c_int, _P, _P, _P,
+              c_int, c_int, c_int, c_int, c_int, c_int]
+_XYFUN_CREATE = _XY_CREATE + [_P]
+
+_VARIANTS = {
+    "2DXnp": _X_CREATE, "2DXp": _X_CREATE,
+    "2DXnpFun": _XFUN_CREATE, "2DXpFun": _XFUN_CREATE,
+    "2DYnp": _X_CREATE, "2DYp": _X_CREATE,
+    "2DYnpFun": _YNPFUN_CREATE, "2DYpFun": _XFUN_CREATE,
+    "2DXYnp": _XY_CREATE, "2DXYp": _XY_CREATE,
+    "2DXYnpFun": _XYFUN_CREATE, "2DXYpFun": _XYFUN_CREATE,
+}
+
+_SIGS: dict[str, tuple] = {}
+for _v, _cargs in _VARIANTS.items():
+    for _pre in ("cuSten", "custen"):
+        _SIGS[f"{_pre}Create{_v}"] = (None, _cargs)
+        _SIGS[f"{_pre}Compute{_v}"] = (None, [_H, c_bool])
+        _SIGS[f"{_pre}Swap{_v}"] = (None, [_H, _P])
+        _SIGS[f"{_pre}Destroy{_v}"] = (None, [_H])
+_SIGS.update({
+    "cuSenCompute2DXpFun": (None, [_H, c_bool]),
+    "checkError": (None, [c_char_p]),
+    "custenGetLastError": (c_int, []),
+    "custenGetLastErrorString": (c_char_p, []),
+    "custenClearError": (None, []),
+    "custenVersion": (c_char_p, []),
+    "custenHandleSize": (c_size_t, []),
+    "custenGetBuiltinFunction": (c_void_p, [c_char_p, c_int]),
+    "custenSetForceIndirect": (None, [_H, c_int]),
+    "custenCreate2DF32": (None, [_H, c_int, c_int, c_int, c_int, c_int, c_int, _P, _P, _P,
+                                 c_int, c_int, c_int, c_int, c_int, c_int]),
+    "custenCompute2DF32": (None, [_H, c_bool]),
+    "custenSwap2DF32": (None, [_H, _P]),
+    "custenDestroy2DF32": (None, [_H]),
+    "custenSetStream": (None, [_H, c_void_p]),
+    "custenSetTileConfig": (None, [_H, c_int, c_int]),
+    "custenBatchCreate": (c_void_p, [c_int, c_int, _P, _P, _P, _P, _P, c_int, c_int, c_int,
+                                     c_int, c_int]),
+    "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "custenBatchParity": (c_int, [c_void_p]),
+    "custenBatchDestroy": (None, [c_void_p]),
+    "custenFillUniform": (None, [_P, c_size_t, c_uint64, c_double, c_double]),
+    "custenCHCreate": (c_void_p, [c_int, ctypes.POINTER(custenCHParams), _P]),
+    "custenCHRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "custenCHGetField": (c_int, [c_void_p, _P, _P]),
+    "custenCHDestroy": (None, [c_void_p]),
+    "custenCHServerCreate": (c_void_p, [c_int, c_size_t, c_int]),
+    "custenCHServerSubmit": (c_int, [c_void_p, ctypes.POINTER(custenCHParams), c_int, _P]),
+    "custenCHServerRunUntilIdle": (c_int, [c_void_p, c_int]),
+    "custenCHServerStep": (c_int, [c_void_p]),
+    "custenCHServerQuery": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int),
+                                    ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    "custenCHServerResult": (c_int, [c_void_p, c_int, _P]),
+    "custenCHFootprint": (c_size_t, [c_int]),
+    "custenCHServerDestroy": (None, [c_void_p]),
+})
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"custen native library not found at {LIB_PATH}; build it with "
+            "`python -m custen_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def header_symbols() -> list[str]:
+    """Every function name declared in include/cuSten.h (aliases expanded)."""
+    text = open(HEADER_PATH).read()
+    body = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"^\s*(?:void|int|size_t|double\*|const char\*|custen\w+\*)\s+"
+                           r"(\w+)\s*\(", body, flags=re.M))
+    alias_list = re.search(r"#define CUSTEN_ALIAS_LIST\(X\)(.*?)\n\n", text, flags=re.S)
+    if alias_list:
+        for suffix in re.findall(r"X\((\w+)\)", alias_list.group(1)):
+            names.add("custen" + suffix)
+    return sorted(names)
+
+
+def _addr(x) -> int | None:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):  # torch tensor
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):  # numpy array
+        return x.ctypes.data
+    if isinstance(x, ctypes._Pointer) or isinstance(x, c_void_p):
+        return ctypes.cast(x, c_void_p).value
+    raise TypeError(f"cannot take the address of {type(x)!r}")
+
+
+def check(what: str = "") -> None:
+    L = lib()
+    code = L.custenGetLastError()
+    if code != 0:
+        msg = L.custenGetLastErrorString().decode()
+        L.custenClearError()
+        raise CustenError(code, f"{what}: {msg}" if what else msg)
+
+
+def clear_error() -> None:
+    lib().custenClearError()
+
+
+def _handle(h: cuSten_t):
+    return ctypes.byref(h)
+
+
+def _make_create(name: str, nptr: int):
+    """Create wrapper: the first three pointer arguments after BLOCK_Y are data
+    pointers; Fun variants end with the function pointer."""
+
+    def create(h: cuSten_t, *args):
+        args = list(args)
+        # positions (after the handle) of pointer arguments: 6, 7, 8 (out, in, weights/coe)
+        for i in (6, 7, 8):
+            args[i] = _addr(args[i])
+        if name.endswith("Fun"):
+            args[-1] = _addr(args[-1])
+        getattr(lib(), name)(_handle(h), *args)
+        check(name)
+
+    create.__name__ = name
+    return create
+
+
+def _make_compute(name):
+    def compute(h: cuSten_t, offload: int = DEVICE):
+        getattr(lib(), name)(_handle(h), bool(offload))
+        check(name)
+
+    compute.__name__ = name
+    return compute
+
+
+def _make_swap(name):
+    def swap(h: cuSten_t, dataInput):
+        getattr(lib(), name)(_handle(h), _addr(dataInput))
+        check(name)
+
+    swap.__name__ = name
+    return swap
+
+
+def _make_destroy(name):
+    def destroy(h: cuSten_t):
+        getattr(lib(), name)(_handle(h))
+        check(name)
+
+    destroy.__name__ = name
+    return destroy
+
+
+for _v in _VARIANTS:
+    for _pre in ("cuSten", "custen"):
+        globals()[f"{_pre}Create{_v}"] = _make_create(f"{_pre}Create{_v}", 3)
+        globals()[f"{_pre}Compute{_v}"] = _make_compute(f"{_pre}Compute{_v}")
+        globals()[f"{_pre}Swap{_v}"] = _make_swap(f"{_pre}Swap{_v}")
+        globals()[f"{_pre}Destroy{_v}"] = _make_destroy(f"{_pre}Destroy{_v}")
+cuSenCompute2DXpFun = _make_compute("cuSenCompute2DXpFun")
+
+
+def custenCreate2DF32(h, deviceNum, numTiles, nx, ny, dir_, periodic, dataOutput, dataInput,
+                      weights, numStenHoriz, numStenLeft, numStenRight, numStenVert, numStenTop,
+                      numStenBottom):
+    lib().custenCreate2DF32(_handle(h), deviceNum, numTiles, nx, ny, dir_, periodic,
+                            _addr(dataOutput), _addr(dataInput), _addr(weights), numStenHoriz,
+                            numStenLeft, numStenRight, numStenVert, numStenTop, numStenBottom)
+    check("custenCreate2DF32")
+
+
+custenCompute2DF32 = _make_compute("custenCompute2DF32")
+custenSwap2DF32 = _make_swap("custenSwap2DF32")
+custenDestroy2DF32 = _make_destroy("custenDestroy2DF32")
+
+
+def custenGetBuiltinFunction(name: str, deviceNum: int = 0) -> int:
+    addr = lib().custenGetBuiltinFunction(name.encode(), deviceNum)
+    check("custenGetBuiltinFunction")
+    return addr
+
+
+def custenSetForceIndirect(h: cuSten_t, force: bool) -> None:
+    lib().custenSetForceIndirect(_handle(h), int(bool(force)))
+    check("custenSetForceIndirect")
+
+
+def custenSetTileConfig(h: cuSten_t, rowsPerWave: int, chunks: int) -> None:
+    lib().custenSetTileConfig(_handle(h), rowsPerWave, chunks)
+    check("custenSetTileConfig")
+
+
+def custenSetStream(h: cuSten_t, stream: int) -> None:
+    lib().custenSetStream(_handle(h), stream)
+    check("custenSetStream")
+
+
+def custenFillUniform(out, n: int, seed: int, a: float, b: float) -> None:
+    lib().custenFillUniform(_addr(out), n, seed, a, b)

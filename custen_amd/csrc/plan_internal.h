@@ -1,0 +1,62 @@
+// plan_internal.h -- library-private state behind cuSten_t::plan and the
+// error/last-error machinery shared by all host entry points.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cuSten.h"
+#include "kernels.h"
+
+namespace custen {
+
+enum class Dir : int { X = 0, Y = 1, XY = 2 };
+
+// Where a data pointer lives, as far as a kernel launch is concerned.
+enum class MemKind : int { Device = 0, Host = 1 };
+
+struct Plan {
+    Dir dir = Dir::XY;
+    bool periodic = true;
+    bool fun = false;
+    Dtype dtype = Dtype::F64;
+    int H = 1, L = 0, V = 1, T = 0;
+    int numCoe = 0;
+    Builtin builtin = Builtin::None;
+    bool forceIndirect = false;
+    bool zeroRight = false;     // x-np: right R columns written as 0.0
+    // float-storage handles keep their float pointers here (the double**
+    // tables in cuSten_t hold the same addresses)
+    hipStream_t userStream = nullptr;
+    bool hasUserStream = false;
+    int rowsPerWave = 0;
+    int chunks = 0;
+    // device copy of host-resident weights / coefficients
+    double* paramDev = nullptr;
+    size_t paramCap = 0;
+    // streaming resources (host-resident data)
+    void* stageIn[2] = {nullptr, nullptr};
+    void* stageOut[2] = {nullptr, nullptr};
+    size_t stageInBytes = 0, stageOutBytes = 0;
+    hipEvent_t evH2D[2] = {nullptr, nullptr};
+    hipEvent_t evComp[2] = {nullptr, nullptr};
+    hipEvent_t evD2H[2] = {nullptr, nullptr};
+    bool streamEventsReady = false;
+    uint32_t magic = 0x43535450u;  // "CSTP"
+};
+
+// ---- errors ----------------------------------------------------------------
+void setError(int code, const std::string& msg);
+void setHipError(hipError_t e, const char* what);
+int lastError();
+
+// Classify a pointer; clears any HIP error the query leaves behind.
+MemKind classify(const void* p, int device);
+
+// Validated access to a handle's plan (records an error and returns null).
+Plan* planOf(cuSten_t* h, const char* fn);
+
+}  // namespace custen

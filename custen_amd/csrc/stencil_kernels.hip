@@ -1,0 +1,528 @@
+// stencil_kernels.hip -- gfx950 stencil kernels behind cuStenCompute2D*.
+//
+//   stencilFixed   : compile-time stencil shape (H x V), register sliding
+//                    window, weights held in registers.  Serves the common
+//                    x / y / xy shapes and the built-in xy point functions.
+//   stencilGeneric : any shape up to 65 x 65, weights staged in LDS.
+//   funKernel      : point-function ("Fun") variants; calls the user's device
+//                    function through its pointer, or an inlined built-in.
+//
+// Semantics follow the reference kernels (cuSten/src/kernels/2d_*_kernel.cu):
+// out(y,x) = fma chain over j < V, i < H of w[j*H+i] * in(y-T+j, x-L+i),
+// starting from 0.0; periodic variants wrap, non-periodic variants write only
+// the points whose whole window lies in the domain (see StencilLaunch).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "kernels.h"
+#include "stencil_tile.h"
+
+#pragma clang fp contract(off)
+
+namespace custen {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Fixed-shape kernel
+// ---------------------------------------------------------------------------
+template <typename T, int H, int V, int RPW, Xform X>
+__global__ __launch_bounds__(kThreads) void stencilFixed(StencilLaunch a)
+{
+    constexpr int TH = kWaves * RPW;
+    constexpr int NR = TH + V - 1;
+    constexpr int LD = ldsRowLength(H);
+    __shared__ __attribute__((aligned(16))) T tile[NR * LD];
+
+    const T* __restrict__ in = static_cast<const T*>(a.in);
+    T* __restrict__ out = static_cast<T*>(a.out);
+
+    T w[H * V];
+#pragma unroll
+    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
+
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int L = a.L;
+    const int R = H - 1 - L;
+    const int LP = (L + 1) & ~1;
+    const int nx = a.nx;
+    const int tilesX = (nx + kTW - 1) / kTW;
+    const int g = xcdRemap(blockIdx.x, gridDim.x);
+    const int tx = g % tilesX;
+    const int tyg = g / tilesX;
+    const int x0 = tx * kTW;
+    const bool inside = x0 + kTW <= nx;
+    const bool vecIn = a.vecIn && inside;
+    const bool fullRow = a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1;
+
+    for (int c = 0; c < a.chunks; c++) {
+        const int y0 = (tyg * a.chunks + c) * TH;
+        if (y0 >= a.ny) break;
+        if (c > 0) __syncthreads();
+        stageTile<T, X>(tile, LD, NR, in, a.ldIn, a.rows, y0, a.T, x0, nx, L, R, LP, a.xWrap,
+                        vecIn, wave, lane);
+        __syncthreads();
+
+        T acc[RPW][2];
+#pragma unroll
+        for (int r = 0; r < RPW; r++) acc[r][0] = acc[r][1] = T(0);
+
+        const T* base = tile + (wave * RPW) * LD + (LP - L) + 2 * lane;
+#pragma unroll
+        for (int ir = 0; ir < RPW + V - 1; ir++) {
+            T v[H + 1];
+#pragma unroll
+            for (int k = 0; k <= H; k++) v[k] = base[ir * LD + k];
+#pragma unroll
+            for (int r = 0; r < RPW; r++) {
+                const int j = ir - r;
+                if (j >= 0 && j < V) {
+#pragma unroll
+                    for (int i = 0; i < H; i++) {
+                        acc[r][0] = fmaT(w[j * H + i], v[i], acc[r][0]);
+                        acc[r][1] = fmaT(w[j * H + i], v[i + 1], acc[r][1]);
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const int gy = y0 + wave * RPW + r;
+            if (gy >= a.ny) break;
+            const int gyg = a.yGlobal0 + gy;
+            if (gyg < a.wy0 || gyg >= a.wy1) continue;
+            storePair<T>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r][0], acc[r][1], fullRow,
+                         nx, a.wx0, a.wx1, a.zeroRight, R);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic-shape kernel (runtime H, V); LDS = tile + weights.
+// ---------------------------------------------------------------------------
+constexpr int kGenericRPW = 4;
+
+template <typename T, Xform X>
+__global__ __launch_bounds__(kThreads) void stencilGeneric(StencilLaunch a)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int RPW = kGenericRPW;
+    constexpr int TH = kWaves * RPW;
+    const int H = a.H, V = a.V;
+    const int NR = TH + V - 1;
+    const int LD = ldsRowLength(H);
+    T* tile = reinterpret_cast<T*>(smem);
+    T* wl = tile + NR * LD;
+
+    const T* __restrict__ in = static_cast<const T*>(a.in);
+    T* __restrict__ out = static_cast<T*>(a.out);
+
+    for (int k = threadIdx.x; k < H * V; k += kThreads) wl[k] = static_cast<T>(a.weights[k]);
+
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int L = a.L;
+    const int R = H - 1 - L;
+    const int LP = (L + 1) & ~1;
+    const int nx = a.nx;
+    const int tilesX = (nx + kTW - 1) / kTW;
+    const int g = xcdRemap(blockIdx.x, gridDim.x);
+    const int tx = g % tilesX;
+    const int tyg = g / tilesX;
+    const int x0 = tx * kTW;
+    const bool inside = x0 + kTW <= nx;
+    const bool vecIn = a.vecIn && inside;
+    const bool fullRow = a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1;
+
+    for (int c = 0; c < a.chunks; c++) {
+        const int y0 = (tyg * a.chunks + c) * TH;
+        if (y0 >= a.ny) break;
+        __syncthreads();
+        stageTile<T, X>(tile, LD, NR, in, a.ldIn, a.rows, y0, a.T, x0, nx, L, R, LP, a.xWrap,
+                        vecIn, wave, lane);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const int gy = y0 + wave * RPW + r;
+            if (gy >= a.ny) break;
+            const int gyg = a.yGlobal0 + gy;
+            if (gyg < a.wy0 || gyg >= a.wy1) continue;
+            T s0 = T(0), s1 = T(0);
+            const T* base = tile + (wave * RPW + r) * LD + (LP - L) + 2 * lane;
+            for (int j = 0; j < V; j++) {
+                const T* rowp = base + j * LD;
+                for (int i = 0; i < H; i++) {
+                    const T wk = wl[j * H + i];
+                    s0 = fmaT(wk, rowp[i], s0);
+                    s1 = fmaT(wk, rowp[i + 1], s1);
+                }
+            }
+            storePair<T>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, s0, s1, fullRow, nx, a.wx0,
+                         a.wx1, a.zeroRight, R);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Point-function kernels
+// ---------------------------------------------------------------------------
+typedef double (*FnX)(double*, double*, int);
+typedef double (*FnY)(double*, double*, int, int);
+typedef double (*FnXY)(double*, double*, int, int, int, int);
+
+constexpr int kFunRPW = 4;
+constexpr int kMaxCoe = 1024;
+
+struct CallIndirect {
+    template <int DIR>
+    __device__ static double call(const void* fn, double* d, double* c, int loc, int jump, int H,
+                                  int V)
+    {
+        if constexpr (DIR == 0) return reinterpret_cast<FnX>(const_cast<void*>(fn))(d, c, loc);
+        else if constexpr (DIR == 1)
+            return reinterpret_cast<FnY>(const_cast<void*>(fn))(d, c, loc, jump);
+        else return reinterpret_cast<FnXY>(const_cast<void*>(fn))(d, c, loc, jump, H, V);
+    }
+};
+
+// Built-in point functions.  The inlined functors and the addressable device
+// functions below share these bodies, so both paths give identical bits.
+__device__ __forceinline__ double bodyXCentralDiff(const double* d, const double* c, int l)
+{
+    double t = d[l - 1] - 2.0 * d[l];
+    t = t + d[l + 1];
+    return t * c[0];
+}
+__device__ __forceinline__ double bodyYCentralDiff9(const double* d, const double* c, int l, int jump)
+{
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r = __builtin_fma(c[i], d[(l - 4 * jump) + i * jump], r);
+    return r;
+}
+__device__ __forceinline__ double bodyXYWeightedSum(const double* d, const double* c, int l,
+                                                    int jump, int nx, int ny)
+{
+    double r = 0.0;
+    int cnt = 0;
+    for (int j = 0; j < ny; j++) {
+        const int t = l + j * jump;
+        for (int i = 0; i < nx; i++) r = __builtin_fma(c[cnt++], d[t + i], r);
+    }
+    return r;
+}
+__device__ __forceinline__ double bodyXYCubeMinus(const double* d, const double* c, int l, int jump,
+                                                  int nx, int ny)
+{
+    double r = 0.0;
+    int cnt = 0;
+    for (int j = 0; j < ny; j++) {
+        const int t = l + j * jump;
+        for (int i = 0; i < nx; i++) {
+            const double u = d[t + i];
+            const double u2 = u * u;
+            const double u3 = u2 * u;
+            r = __builtin_fma(c[cnt++], u3 - u, r);
+        }
+    }
+    return r;
+}
+
+struct CallXCentralDiff {
+    template <int DIR>
+    __device__ static double call(const void*, double* d, double* c, int loc, int, int, int)
+    {
+        return bodyXCentralDiff(d, c, loc);
+    }
+};
+struct CallYCentralDiff9 {
+    template <int DIR>
+    __device__ static double call(const void*, double* d, double* c, int loc, int jump, int, int)
+    {
+        return bodyYCentralDiff9(d, c, loc, jump);
+    }
+};
+
+template <int DIR, class F>
+__global__ __launch_bounds__(kThreads) void funKernel(FunLaunch a)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int RPW = kFunRPW;
+    constexpr int TH = kWaves * RPW;
+    const int H = a.H, V = a.V;
+    const int NR = TH + V - 1;
+    const int LD = ldsRowLength(H);
+    double* tile = reinterpret_cast<double*>(smem);
+    double* coeL = tile + NR * LD;
+
+    for (int k = threadIdx.x; k < a.numCoe; k += kThreads) coeL[k] = a.coe[k];
+
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int L = a.L;
+    const int R = H - 1 - L;
+    const int LP = (L + 1) & ~1;
+    const int nx = a.nx;
+    const int tilesX = (nx + kTW - 1) / kTW;
+    const int g = xcdRemap(blockIdx.x, gridDim.x);
+    const int tx = g % tilesX;
+    const int tyg = g / tilesX;
+    const int x0 = tx * kTW;
+    const bool vecIn = a.vecIn && (x0 + kTW <= nx);
+
+    for (int c = 0; c < a.chunks; c++) {
+        const int y0 = (tyg * a.chunks + c) * TH;
+        if (y0 >= a.ny) break;
+        __syncthreads();
+        stageTile<double, Xform::Identity>(tile, LD, NR, a.in, a.ldIn, a.rows, y0, a.T, x0, nx, L,
+                                           R, LP, a.xWrap, vecIn, wave, lane);
+        __syncthreads();
+        for (int r = 0; r < RPW; r++) {
+            const int gy = y0 + wave * RPW + r;
+            if (gy >= a.ny) break;
+            const int gyg = a.yGlobal0 + gy;
+            if (gyg < a.wy0 || gyg >= a.wy1) continue;
+            const int rr = wave * RPW + r;
+            for (int e = 0; e < 2; e++) {
+                const int gx = x0 + 2 * lane + e;
+                if (gx >= nx || gx < a.wx0 || gx >= a.wx1) continue;
+                // window top-left in LDS
+                const int tl = rr * LD + (LP - L) + 2 * lane + e;
+                int loc;
+                if constexpr (DIR == 0) loc = tl + L;           // centre (V == 1)
+                else if constexpr (DIR == 1) loc = tl + a.T * LD;  // centre (H == 1)
+                else loc = tl;                                  // top-left
+                const double val = F::template call<DIR>(a.func, tile, coeL, loc, LD, H, V);
+                a.out[(int64_t)gy * a.ldOut + gx] = val;
+            }
+        }
+    }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// Addressable built-in functions (custenGetBuiltinFunction)
+// ---------------------------------------------------------------------------
+__device__ double custenBuiltinXCentralDifference(double* d, double* c, int loc)
+{
+    return dev::bodyXCentralDiff(d, c, loc);
+}
+__device__ double custenBuiltinYCentralDifference9(double* d, double* c, int loc, int jump)
+{
+    return dev::bodyYCentralDiff9(d, c, loc, jump);
+}
+__device__ double custenBuiltinXYWeightedSum(double* d, double* c, int loc, int jump, int nx, int ny)
+{
+    return dev::bodyXYWeightedSum(d, c, loc, jump, nx, ny);
+}
+__device__ double custenBuiltinXYCubeMinus(double* d, double* c, int loc, int jump, int nx, int ny)
+{
+    return dev::bodyXYCubeMinus(d, c, loc, jump, nx, ny);
+}
+
+__device__ dev::FnX custenPtrXCentralDifference = custenBuiltinXCentralDifference;
+__device__ dev::FnY custenPtrYCentralDifference9 = custenBuiltinYCentralDifference9;
+__device__ dev::FnXY custenPtrXYWeightedSum = custenBuiltinXYWeightedSum;
+__device__ dev::FnXY custenPtrXYCubeMinus = custenBuiltinXYCubeMinus;
+
+hipError_t builtinFunctionAddress(Builtin b, void** addr)
+{
+    *addr = nullptr;
+    switch (b) {
+    case Builtin::XCentralDiff:
+        return hipMemcpyFromSymbol(addr, HIP_SYMBOL(custenPtrXCentralDifference), sizeof(void*));
+    case Builtin::YCentralDiff9:
+        return hipMemcpyFromSymbol(addr, HIP_SYMBOL(custenPtrYCentralDifference9), sizeof(void*));
+    case Builtin::XYWeightedSum:
+        return hipMemcpyFromSymbol(addr, HIP_SYMBOL(custenPtrXYWeightedSum), sizeof(void*));
+    case Builtin::XYCubeMinus:
+        return hipMemcpyFromSymbol(addr, HIP_SYMBOL(custenPtrXYCubeMinus), sizeof(void*));
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kDefaultRPW = 8;
+constexpr int kDefaultChunks = 4;
+
+template <typename T>
+bool pairAligned(const void* p, int64_t ld)
+{
+    return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T)) == 0) && (ld % 2 == 0);
+}
+
+int gridSize(int nx, int ny, int TH, int chunks)
+{
+    const int tilesX = (nx + dev::kTW - 1) / dev::kTW;
+    const int rowsPerGroup = TH * chunks;
+    const int tilesY = (ny + rowsPerGroup - 1) / rowsPerGroup;
+    return tilesX * tilesY;
+}
+
+template <typename T, int H, int V, int RPW, Xform X>
+hipError_t runFixed(const StencilLaunch& s, hipStream_t st)
+{
+    const int grid = gridSize(s.nx, s.ny, dev::kWaves * RPW, s.chunks);
+    hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X>), dim3(grid), dim3(dev::kThreads), 0,
+                       st, s);
+    return hipGetLastError();
+}
+
+template <typename T, Xform X>
+hipError_t runGeneric(const StencilLaunch& s, hipStream_t st)
+{
+    const int TH = dev::kWaves * dev::kGenericRPW;
+    const int grid = gridSize(s.nx, s.ny, TH, s.chunks);
+    const size_t lds =
+        sizeof(T) * ((size_t)(TH + s.V - 1) * dev::ldsRowLength(s.H) + (size_t)s.H * s.V);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dev::stencilGeneric<T, X>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((dev::stencilGeneric<T, X>), dim3(grid), dim3(dev::kThreads), lds, st, s);
+    return hipGetLastError();
+}
+
+// Shapes with a compiled fixed kernel: x: H in {3,5,7,9}; y: V in {3,5,7,9};
+// xy: 3x3, 5x5.  RPW in {4, 8}.
+#define CUSTEN_FIXED_SHAPES(X_) \
+    X_(3, 1) X_(5, 1) X_(7, 1) X_(9, 1) X_(1, 3) X_(1, 5) X_(1, 7) X_(1, 9) X_(3, 3) X_(5, 5)
+
+template <typename T, Xform X>
+hipError_t dispatchTyped(const StencilLaunch& s, hipStream_t st)
+{
+    const int rpw = s.rowsPerWave;
+#define CUSTEN_CASE(HH, VV)                                                      \
+    if (s.H == HH && s.V == VV) {                                                \
+        if (rpw == 4) return runFixed<T, HH, VV, 4, X>(s, st);                   \
+        return runFixed<T, HH, VV, 8, X>(s, st);                                 \
+    }
+    CUSTEN_FIXED_SHAPES(CUSTEN_CASE)
+#undef CUSTEN_CASE
+    return runGeneric<T, X>(s, st);
+}
+
+}  // namespace
+
+int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave)
+{
+    const int es = dt == Dtype::F64 ? 8 : 4;
+    const int rpw = rowsPerWave > 0 ? rowsPerWave : kDefaultRPW;
+    return es * (dev::kWaves * rpw + V - 1) * dev::ldsRowLength(H);
+}
+
+hipError_t launchStencil(const StencilLaunch& in, hipStream_t st)
+{
+    StencilLaunch s = in;
+    if (s.nx <= 0 || s.ny <= 0) return hipSuccess;
+    if (s.H < 1 || s.V < 1 || s.H > 65 || s.V > 65) return hipErrorInvalidValue;
+    if (s.rowsPerWave != 4 && s.rowsPerWave != 8) s.rowsPerWave = kDefaultRPW;
+    if (s.chunks <= 0) s.chunks = kDefaultChunks;
+    if (s.dtype == Dtype::F64) {
+        s.vecIn = pairAligned<double>(s.in, s.ldIn);
+        s.vecOut = pairAligned<double>(s.out, s.ldOut);
+        if (s.xform == Xform::CubeMinus) return dispatchTyped<double, Xform::CubeMinus>(s, st);
+        return dispatchTyped<double, Xform::Identity>(s, st);
+    }
+    s.vecIn = pairAligned<float>(s.in, s.ldIn);
+    s.vecOut = pairAligned<float>(s.out, s.ldOut);
+    if (s.xform == Xform::CubeMinus) return hipErrorInvalidValue;
+    return dispatchTyped<float, Xform::Identity>(s, st);
+}
+
+hipError_t launchFun(const FunLaunch& in, hipStream_t st)
+{
+    FunLaunch f = in;
+    if (f.nx <= 0 || f.ny <= 0) return hipSuccess;
+    if (f.H < 1 || f.V < 1 || f.H > 65 || f.V > 65) return hipErrorInvalidValue;
+    if (f.numCoe < 0 || f.numCoe > dev::kMaxCoe) return hipErrorInvalidValue;
+    if (f.chunks <= 0) f.chunks = kDefaultChunks;
+    f.vecIn = pairAligned<double>(f.in, f.ldIn);
+
+    // Built-in xy functions are weighted sums (of u or of u^3 - u): run them on
+    // the register-blocked stencil kernel with the coefficients as weights.
+    if (f.dir == 2 && (f.builtin == Builtin::XYWeightedSum || f.builtin == Builtin::XYCubeMinus) &&
+        f.numCoe >= f.H * f.V) {
+        StencilLaunch s{};
+        s.dtype = Dtype::F64;
+        s.in = f.in;
+        s.out = f.out;
+        s.weights = f.coe;
+        s.ldIn = f.ldIn;
+        s.ldOut = f.ldOut;
+        s.nx = f.nx;
+        s.ny = f.ny;
+        s.H = f.H;
+        s.L = f.L;
+        s.V = f.V;
+        s.T = f.T;
+        s.xWrap = f.xWrap;
+        s.rows = f.rows;
+        s.wx0 = f.wx0;
+        s.wx1 = f.wx1;
+        s.wy0 = f.wy0;
+        s.wy1 = f.wy1;
+        s.yGlobal0 = f.yGlobal0;
+        s.zeroRight = 0;
+        s.xform = f.builtin == Builtin::XYCubeMinus ? Xform::CubeMinus : Xform::Identity;
+        s.rowsPerWave = f.rowsPerWave;
+        s.chunks = f.chunks;
+        return launchStencil(s, st);
+    }
+
+    const int TH = dev::kWaves * dev::kFunRPW;
+    const int grid = gridSize(f.nx, f.ny, TH, f.chunks);
+    const size_t lds = sizeof(double) * ((size_t)(TH + f.V - 1) * dev::ldsRowLength(f.H) +
+                                         (size_t)(f.numCoe > 0 ? f.numCoe : 1));
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        const void* kernels[] = {reinterpret_cast<const void*>(&dev::funKernel<0, dev::CallXCentralDiff>),
+                                 reinterpret_cast<const void*>(&dev::funKernel<1, dev::CallYCentralDiff9>),
+                                 reinterpret_cast<const void*>(&dev::funKernel<0, dev::CallIndirect>),
+                                 reinterpret_cast<const void*>(&dev::funKernel<1, dev::CallIndirect>),
+                                 reinterpret_cast<const void*>(&dev::funKernel<2, dev::CallIndirect>)};
+        for (const void* k : kernels) {
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+    }
+
+    if (f.builtin == Builtin::XCentralDiff && f.dir == 0) {
+        hipLaunchKernelGGL((dev::funKernel<0, dev::CallXCentralDiff>), dim3(grid),
+                           dim3(dev::kThreads), lds, st, f);
+        return hipGetLastError();
+    }
+    if (f.builtin == Builtin::YCentralDiff9 && f.dir == 1) {
+        hipLaunchKernelGGL((dev::funKernel<1, dev::CallYCentralDiff9>), dim3(grid),
+                           dim3(dev::kThreads), lds, st, f);
+        return hipGetLastError();
+    }
+    if (f.func == nullptr) return hipErrorInvalidValue;
+    switch (f.dir) {
+    case 0:
+        hipLaunchKernelGGL((dev::funKernel<0, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
+                           lds, st, f);
+        break;
+    case 1:
+        hipLaunchKernelGGL((dev::funKernel<1, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
+                           lds, st, f);
+        break;
+    default:
+        hipLaunchKernelGGL((dev::funKernel<2, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
+                           lds, st, f);
+        break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace custen

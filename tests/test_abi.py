@@ -1,0 +1,64 @@
+"""The C-ABI library loads, exports every entry point include/cuSten.h
+declares (the reference's 48 Create/Compute/Swap/Destroy + the misspelt
+cuSenCompute2DXpFun + checkError, the lower-case aliases and the extensions),
+and validates arguments before touching the GPU."""
+import ctypes
+
+import pytest
+
+from custen_amd import api
+
+
+def test_library_loads_and_exports_header_symbols():
+    L = api.lib()
+    names = api.header_symbols()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, f"not exported: {missing}"
+    # the reference surface: 12 variants x 4 entry points
+    ref = [f"cuSten{op}2D{v}" for op in ("Create", "Compute", "Swap", "Destroy")
+           for v in ("Xp", "Xnp", "XpFun", "XnpFun", "Yp", "Ynp", "YpFun", "YnpFun",
+                     "XYp", "XYnp", "XYpFun", "XYnpFun")]
+    assert len(ref) == 48 and all(n in names for n in ref)
+    assert "cuSenCompute2DXpFun" in names and "checkError" in names
+
+
+def test_aliases_share_addresses():
+    L = api.lib()
+    for v in ("Create2DXYp", "Compute2DXYpFun", "Swap2DYnp", "Destroy2DXnpFun"):
+        a = ctypes.cast(getattr(L, "cuSten" + v), ctypes.c_void_p).value
+        b = ctypes.cast(getattr(L, "custen" + v), ctypes.c_void_p).value
+        assert a == b
+
+
+def test_handle_size_matches_binding():
+    assert api.lib().custenHandleSize() == ctypes.sizeof(api.cuSten_t)
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((0, 1, 64, 64, 8, 8, 1, 2, 3, 5, 2, 1, 5, 2, 2), "horizontal stencil size"),
+    ((0, 3, 64, 64, 8, 8, 1, 2, 3, 5, 2, 2, 5, 2, 2), "divisible by numTiles"),
+    ((0, 1, 0, 64, 8, 8, 1, 2, 3, 5, 2, 2, 5, 2, 2), "nx and ny"),
+    ((0, 1, 64, 64, 8, 8, 1, 1, 3, 5, 2, 2, 5, 2, 2), "alias"),
+    ((0, 1, 64, 64, 8, 8, 1, 2, None, 5, 2, 2, 5, 2, 2), "weights"),
+    ((0, 1, 64, 64, 8, 8, 1, 2, 3, 5, -1, 5, 5, 2, 2), "extents"),
+])
+def test_create_validation(args, msg):
+    h = api.cuSten_t()
+    with pytest.raises(api.CustenError) as ei:
+        api.cuStenCreate2DXYp(h, *args)
+    assert msg in str(ei.value)
+    assert ei.value.code == 1
+    assert not h.plan
+
+
+def test_uncreated_handle_is_rejected():
+    h = api.cuSten_t()
+    with pytest.raises(api.CustenError):
+        api.cuStenCompute2DXYp(h, api.DEVICE)
+    with pytest.raises(api.CustenError):
+        api.cuStenDestroy2DXYp(h)
+
+
+def test_builtin_names():
+    with pytest.raises(api.CustenError):
+        api.custenGetBuiltinFunction("no_such_function", 0)
