@@ -27,7 +27,8 @@ HOST = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libcusten.so")
-HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cuSten.h")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+HEADER_PATH = os.path.join(INCLUDE_DIR, "cuSten.h")
 
 c_int = ctypes.c_int
 c_bool = ctypes.c_bool
@@ -145,7 +146,7 @@ _SIGS.update({
     "custenSetStream": (None, [_H, c_void_p]),
     "custenSetTileConfig": (None, [_H, c_int, c_int]),
     "custenBatchCreate": (c_void_p, [c_int, c_int, _P, _P, _P, _P, _P, c_int, c_int, c_int,
-                                     c_int, c_int]),
+                                     c_int, c_int]),  # noqa: E501
     "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "custenBatchParity": (c_int, [c_void_p]),
     "custenBatchDestroy": (None, [c_void_p]),
@@ -189,15 +190,19 @@ def lib() -> ctypes.CDLL:
 
 
 def header_symbols() -> list[str]:
-    """Every function name declared in include/cuSten.h (aliases expanded)."""
-    text = open(HEADER_PATH).read()
-    body = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = set(re.findall(r"^\s*(?:void|int|size_t|double\*|const char\*|custen\w+\*)\s+"
-                           r"(\w+)\s*\(", body, flags=re.M))
-    alias_list = re.search(r"#define CUSTEN_ALIAS_LIST\(X\)(.*?)\n\n", text, flags=re.S)
-    if alias_list:
-        for suffix in re.findall(r"X\((\w+)\)", alias_list.group(1)):
-            names.add("custen" + suffix)
+    """Every function name declared in include/*.h (aliases expanded)."""
+    names = set()
+    for fname in sorted(os.listdir(INCLUDE_DIR)):
+        if not fname.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE_DIR, fname)).read()
+        body = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"^\s*(?:void|int|size_t|double\*|const char\*|custen\w+\*)\s+"
+                                r"(\w+)\s*\(", body, flags=re.M))
+        alias_list = re.search(r"#define CUSTEN_ALIAS_LIST\(X\)(.*?)\n\n", text, flags=re.S)
+        if alias_list:
+            for suffix in re.findall(r"X\((\w+)\)", alias_list.group(1)):
+                names.add("custen" + suffix)
     return sorted(names)
 
 
@@ -323,3 +328,154 @@ def custenSetStream(h: cuSten_t, stream: int) -> None:
 
 def custenFillUniform(out, n: int, seed: int, a: float, b: float) -> None:
     lib().custenFillUniform(_addr(out), n, seed, a, b)
+
+
+# ---------------------------------------------------------------------------
+# Cahn-Hilliard ADI (include/custen_ch.h)
+# ---------------------------------------------------------------------------
+def ch_params(n: int, lx: float = 2 * 3.141592653589793, D: float = 1.0, gamma: float = 0.01,
+              dt_over_dx: float = 0.1) -> custenCHParams:
+    import math
+
+    if lx == 2 * 3.141592653589793:
+        lx = 2 * math.pi
+    return custenCHParams(n, lx, D, gamma, dt_over_dx)
+
+
+class CHSolver:
+    """One resident Cahn-Hilliard simulation (custenCH*)."""
+
+    def __init__(self, params: custenCHParams, c0, device: int = 0):
+        self.n = params.n
+        self._p = params
+        self.h = lib().custenCHCreate(device, ctypes.byref(self._p), _addr(c0))
+        if not self.h:
+            check("custenCHCreate")
+            raise CustenError(2, "custenCHCreate failed")
+
+    def run(self, steps: int, mode: int = 0, stream: int = 0) -> None:
+        rc = lib().custenCHRun(self.h, steps, mode, stream)
+        if rc != 0:
+            check("custenCHRun")
+
+    def field(self):
+        import numpy as np
+
+        c = np.empty((self.n, self.n))
+        co = np.empty((self.n, self.n))
+        rc = lib().custenCHGetField(self.h, _addr(c), _addr(co))
+        if rc != 0:
+            check("custenCHGetField")
+        return c, co
+
+    def close(self) -> None:
+        if self.h:
+            lib().custenCHDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CHServer:
+    """Continuous-batching request server (custenCHServer*)."""
+
+    def __init__(self, budget_bytes: int, max_slots: int, device: int = 0):
+        self.h = lib().custenCHServerCreate(device, budget_bytes, max_slots)
+        if not self.h:
+            check("custenCHServerCreate")
+        self._n = {}
+
+    def submit(self, params: custenCHParams, steps: int, c0) -> int:
+        rid = lib().custenCHServerSubmit(self.h, ctypes.byref(params), steps, _addr(c0))
+        if rid < 0:
+            check("custenCHServerSubmit")
+        self._n[rid] = params.n
+        return rid
+
+    def step(self) -> int:
+        left = lib().custenCHServerStep(self.h)
+        if left < 0:
+            check("custenCHServerStep")
+        return left
+
+    def run_until_idle(self, max_iterations: int = 0) -> int:
+        it = lib().custenCHServerRunUntilIdle(self.h, max_iterations)
+        if it < 0:
+            check("custenCHServerRunUntilIdle")
+        return it
+
+    def query(self, rid: int):
+        st = c_int()
+        ls = c_double()
+        la = c_double()
+        if lib().custenCHServerQuery(self.h, rid, ctypes.byref(st), ctypes.byref(ls),
+                                     ctypes.byref(la)) != 0:
+            check("custenCHServerQuery")
+        return st.value, ls.value, la.value
+
+    def result(self, rid: int):
+        import numpy as np
+
+        n = self._n[rid]
+        out = np.empty((n, n))
+        if lib().custenCHServerResult(self.h, rid, _addr(out)) != 0:
+            check("custenCHServerResult")
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            lib().custenCHServerDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ch_footprint(n: int) -> int:
+    return lib().custenCHFootprint(n)
+
+
+# ---------------------------------------------------------------------------
+# Batched grids (include/custen_batch.h)
+# ---------------------------------------------------------------------------
+class Batch:
+    """Many periodic xy grids advanced together (custenBatch*)."""
+
+    def __init__(self, outs, ins, weights, H=5, L=2, V=5, T=2, f32=False, device=0):
+        n = len(ins)
+        self._keep = (outs, ins, weights)
+        nx = (c_int * n)(*[int(t.shape[1]) for t in ins])
+        ny = (c_int * n)(*[int(t.shape[0]) for t in ins])
+        po = (c_void_p * n)(*[_addr(t) for t in outs])
+        pi = (c_void_p * n)(*[_addr(t) for t in ins])
+        self.h = lib().custenBatchCreate(device, n, nx, ny, po, pi, _addr(weights), H, L, V, T,
+                                         1 if f32 else 0)
+        if not self.h:
+            check("custenBatchCreate")
+            raise CustenError(2, "custenBatchCreate failed")
+
+    def run(self, steps: int, mode: int = 0, stream: int = 0) -> None:
+        if lib().custenBatchRun(self.h, steps, mode, stream) != 0:
+            check("custenBatchRun")
+
+    @property
+    def parity(self) -> int:
+        return lib().custenBatchParity(self.h)
+
+    def close(self) -> None:
+        if self.h:
+            lib().custenBatchDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,680 @@
+// ch.cpp -- Cahn-Hilliard ADI on the GPU: per-request setup, a batched step
+// engine, the single-simulation API (custenCH*) and the request server with
+// continuous batching (custenCHServer*).
+//
+// Setup restates cuPentCahnADI/src/cuPentCahnADI.cu:204-230, 368-446 and
+// BatchHyper.cu:24-514 (serial form: serialCahnADI.c:40-417, 776-970):
+// sigma = 2 dt D gamma / (3 dx^4), diagonals [s, -4s, 1+6s, -4s, s], the
+// factorisation of the leading (n-2) block, inv1 = E^-1 f1, inv2 = E^-1 f2 and
+// omega = the inverse of the 2x2 Schur complement.  All n systems of one grid
+// share one matrix, so the factored diagonals are stored once per request
+// (7 (n-2) doubles) instead of replicated per system as the reference does.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cuSten.h"
+#include "../../include/custen_ch.h"
+#include "ch_kernels.h"
+#include "plan_internal.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace custen {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct CHCoeffs {
+    int n = 0, m = 0;
+    double dx = 0, dt = 0, sl = 0, snl = 0;
+    double a = 0, b = 0, c = 0, d = 0, e = 0;
+    double omega[4] = {0, 0, 0, 0};
+    double wlin[25];
+    double wnl[9];
+    std::vector<double> fac;  // ds, dl, d, du, dw, inv1, inv2
+};
+
+void pentFactor(double* ds, double* dl, double* d, double* du, double* dw, int m)
+{
+    // serialCahnADI.c:91-155 / cuPentBatch.cu:35-113
+    int r = 0;
+    du[r] = du[r] / d[r];
+    dw[r] = dw[r] / d[r];
+    int rp = r;
+    r += 1;
+    d[r] = d[r] - dl[r] * du[rp];
+    du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
+    dw[r] = dw[r] / d[r];
+    for (int i = 2; i < m - 2; i++) {
+        const int rpp = r - 1;
+        rp = r;
+        r += 1;
+        dl[r] = dl[r] - ds[r] * du[rpp];
+        d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
+        dw[r] = dw[r] / d[r];
+        du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
+    }
+    for (int last = 0; last < 2; last++) {
+        const int rpp = r - 1;
+        rp = r;
+        r += 1;
+        dl[r] = dl[r] - ds[r] * du[rpp];
+        d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
+        if (last == 0) du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
+    }
+}
+
+void pentSolve(const double* ds, const double* dl, const double* d, const double* du,
+               const double* dw, double* b, int m)
+{
+    // serialCahnADI.c:161-241 / cuPentBatch.cu:119-198
+    b[0] = b[0] / d[0];
+    b[1] = (b[1] - dl[1] * b[0]) / d[1];
+    for (int i = 2; i < m; i++) b[i] = (b[i] - ds[i] * b[i - 2] - dl[i] * b[i - 1]) / d[i];
+    b[m - 2] = b[m - 2] - du[m - 2] * b[m - 1];
+    for (int i = m - 3; i >= 0; i--) b[i] = b[i] - du[i] * b[i + 1] - dw[i] * b[i + 2];
+}
+
+bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
+{
+    if (p.n < 8 || p.n > kCHMaxN) {
+        why = "n must be in [8, " + std::to_string(kCHMaxN) + "]";
+        return false;
+    }
+    if (!(p.lx > 0) || !(p.dtOverDx > 0) || !std::isfinite(p.D) || !std::isfinite(p.gamma)) {
+        why = "lx and dtOverDx must be positive, D and gamma finite";
+        return false;
+    }
+    const int n = p.n, m = n - 2;
+    co.n = n;
+    co.m = m;
+    co.dx = p.lx / n;
+    co.dt = p.dtOverDx * co.dx;
+    const double dx = co.dx, dt = co.dt;
+    co.sl = 2.0 * dt * p.D * p.gamma / (3.0 * (dx * dx * dx * dx));
+    co.snl = (dt / 3.0) * p.D * (2.0 / (dx * dx));
+    const double s = co.sl;
+    co.a = s;
+    co.b = -4 * s;
+    co.c = 1 + 6 * s;
+    co.d = -4 * s;
+    co.e = s;
+    static const double lin[25] = {0, 0, -1, 0, 0,  0, -2, 8, -2, 0,  -1, 8, -20, 8, -1,
+                                   0, -2, 8, -2, 0, 0, 0, -1, 0, 0};
+    static const double nl[9] = {0, 1, 0, 1, -4, 1, 0, 1, 0};
+    for (int k = 0; k < 25; k++) co.wlin[k] = lin[k] * s;
+    for (int k = 0; k < 9; k++) co.wnl[k] = nl[k] * co.snl;
+    co.fac.assign((size_t)7 * m, 0.0);
+    double* ds = co.fac.data();
+    double* dl = ds + m;
+    double* dd = dl + m;
+    double* du = dd + m;
+    double* dw = du + m;
+    double* inv1 = dw + m;
+    double* inv2 = inv1 + m;
+    for (int i = 0; i < m; i++) {
+        if (i > 1) ds[i] = co.a;
+        if (i > 0) dl[i] = co.b;
+        dd[i] = co.c;
+        if (i < m - 1) du[i] = co.d;
+        if (i < m - 2) dw[i] = co.e;
+    }
+    pentFactor(ds, dl, dd, du, dw, m);
+    inv1[0] = co.a;
+    inv1[m - 2] = co.e;
+    inv1[m - 1] = co.d;
+    inv2[0] = co.b;
+    inv2[1] = co.a;
+    inv2[m - 1] = co.e;
+    pentSolve(ds, dl, dd, du, dw, inv1, m);
+    pentSolve(ds, dl, dd, du, dw, inv2, m);
+    const double a = co.a, b = co.b, c = co.c, d = co.d, e = co.e;
+    const double z11 = e * inv1[0] + a * inv1[m - 2] + b * inv1[m - 1];
+    const double z12 = e * inv2[0] + a * inv2[m - 2] + b * inv2[m - 1];
+    const double z21 = d * inv1[0] + e * inv1[1] + a * inv1[m - 1];
+    const double z22 = d * inv2[0] + e * inv2[1] + a * inv2[m - 1];
+    const double y11 = c - z11, y12 = d - z12, y21 = b - z21, y22 = c - z22;
+    const double det = 1.0 / (y11 * y22 - y21 * y12);
+    co.omega[0] = y22 * det;
+    co.omega[1] = -y12 * det;
+    co.omega[2] = -y21 * det;
+    co.omega[3] = y11 * det;
+    return true;
+}
+
+// Device buffers of one request, recycled by grid size.
+struct CHBuffers {
+    int n = 0;
+    double* f[2] = {nullptr, nullptr};
+    double* X = nullptr;
+    double* fac = nullptr;
+};
+
+size_t footprintBytes(int n)
+{
+    const size_t nn = (size_t)n * n;
+    return 3 * nn * sizeof(double) + (size_t)7 * (n - 2) * sizeof(double);
+}
+
+class BufferPool {
+public:
+    explicit BufferPool(size_t keepBytes) : keep_(keepBytes) {}
+    ~BufferPool()
+    {
+        for (auto& kv : free_)
+            for (auto& b : kv.second) release(b);
+    }
+    hipError_t get(int n, CHBuffers& out)
+    {
+        auto it = free_.find(n);
+        if (it != free_.end() && !it->second.empty()) {
+            out = it->second.back();
+            it->second.pop_back();
+            held_ -= footprintBytes(n);
+            return hipSuccess;
+        }
+        out = CHBuffers{};
+        out.n = n;
+        const size_t nn = (size_t)n * n;
+        hipError_t e;
+        if ((e = hipMalloc(&out.f[0], nn * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc(&out.f[1], nn * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc(&out.X, nn * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc(&out.fac, (size_t)7 * (n - 2) * sizeof(double))) != hipSuccess) {
+            release(out);
+            return e;
+        }
+        return hipSuccess;
+    }
+    void put(const CHBuffers& b)
+    {
+        if (held_ + footprintBytes(b.n) > keep_) {
+            CHBuffers c = b;
+            release(c);
+            return;
+        }
+        free_[b.n].push_back(b);
+        held_ += footprintBytes(b.n);
+    }
+
+private:
+    static void release(CHBuffers& b)
+    {
+        for (double* p : {b.f[0], b.f[1], b.X, b.fac})
+            if (p) (void)hipFree(p);
+        b.f[0] = b.f[1] = b.X = b.fac = nullptr;
+    }
+    size_t keep_;
+    size_t held_ = 0;
+    std::map<int, std::vector<CHBuffers>> free_;
+};
+
+struct CHRequest {
+    int id = -1;
+    custenCHParams params{};
+    CHCoeffs co;
+    int steps = 0;
+    int done = 0;
+    int admitStep = 0;
+    CHBuffers buf;
+    std::vector<double> c0;
+    std::vector<double> result;
+    int state = 0;  // 0 queued, 1 resident, 2 finished
+    Clock::time_point tSubmit, tAdmit, tFinish;
+};
+
+// Advances every resident request by one step with one launch per phase.
+class CHEngine {
+public:
+    explicit CHEngine(int device) : device_(device) {}
+    ~CHEngine()
+    {
+        if (dDescs_) (void)hipFree(dDescs_);
+        if (hDescs_) (void)hipHostFree(hDescs_);
+    }
+
+    int step() const { return step_; }
+
+    hipError_t admit(CHRequest* r, const double* c0, hipStream_t st)
+    {
+        const size_t bytes = (size_t)r->co.n * r->co.n * sizeof(double);
+        hipError_t e;
+        if ((e = hipMemcpyAsync(r->buf.f[0], c0, bytes, hipMemcpyDefault, st)) != hipSuccess) return e;
+        // cOld starts equal to c (cuPentCahnADI.cu:295-305)
+        if ((e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(r->buf.fac, r->co.fac.data(), r->co.fac.size() * sizeof(double),
+                                hipMemcpyHostToDevice, st)) != hipSuccess)
+            return e;
+        r->admitStep = step_;
+        active_.push_back(r);
+        dirty_ = true;
+        return hipSuccess;
+    }
+
+    void remove(CHRequest* r)
+    {
+        active_.erase(std::remove(active_.begin(), active_.end(), r), active_.end());
+        dirty_ = true;
+    }
+
+    const std::vector<CHRequest*>& active() const { return active_; }
+
+    // Current field of a resident request.
+    double* current(const CHRequest* r) const { return r->buf.f[(step_ - r->admitStep) & 1]; }
+    double* previous(const CHRequest* r) const
+    {
+        return r->buf.f[((step_ - r->admitStep) & 1) ^ 1];
+    }
+
+    // One step of all resident requests.  plain = true synchronises the stream
+    // after every launch (the reference's launch-and-synchronise pattern).
+    hipError_t stepAll(hipStream_t st, bool plain)
+    {
+        if (active_.empty()) return hipSuccess;
+        hipError_t e;
+        if (dirty_ && (e = upload(st)) != hipSuccess) return e;
+        auto sync = [&]() { return plain ? hipStreamSynchronize(st) : hipSuccess; };
+        if ((e = chLaunchRhs(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        if ((e = chLaunchXSolve(dDescs_, count_, rowBlocks_, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        if ((e = chLaunchYSolve(dDescs_, count_, colBlocks_, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        if ((e = chLaunchFinal(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        step_++;
+        return hipSuccess;
+    }
+
+private:
+    hipError_t upload(hipStream_t st)
+    {
+        const int n = (int)active_.size();
+        if (n > cap_) {
+            // the previous descriptor upload may still be read by queued launches
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            if (dDescs_) (void)hipFree(dDescs_);
+            if (hDescs_) (void)hipHostFree(hDescs_);
+            dDescs_ = nullptr;
+            hDescs_ = nullptr;
+            cap_ = std::max(n, 2 * cap_);
+            if ((e = hipMalloc(&dDescs_, sizeof(CHDesc) * cap_)) != hipSuccess) return e;
+            if ((e = hipHostMalloc(&hDescs_, sizeof(CHDesc) * cap_, 0)) != hipSuccess) return e;
+        } else {
+            // pinned staging is reused: wait until the previous copy consumed it
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+        }
+        int tiles = 0, rows = 0, cols = 0;
+        for (int i = 0; i < n; i++) {
+            const CHRequest* r = active_[i];
+            CHDesc& d = hDescs_[i];
+            std::memset(&d, 0, sizeof(d));
+            d.f[0] = r->buf.f[0];
+            d.f[1] = r->buf.f[1];
+            d.X = r->buf.X;
+            d.fac = r->buf.fac;
+            d.a = r->co.a;
+            d.b = r->co.b;
+            d.d = r->co.d;
+            d.e = r->co.e;
+            for (int k = 0; k < 4; k++) d.omega[k] = r->co.omega[k];
+            for (int k = 0; k < 25; k++) d.wlin[k] = r->co.wlin[k];
+            for (int k = 0; k < 9; k++) d.wnl[k] = r->co.wnl[k];
+            d.n = r->co.n;
+            d.admitStep = r->admitStep;
+            d.tileBase = tiles;
+            d.rowBase = rows;
+            d.colBase = cols;
+            tiles += chTiles(d.n);
+            rows += chRowBlocks(d.n);
+            cols += chColBlocks(d.n);
+        }
+        hipError_t e = hipMemcpyAsync(dDescs_, hDescs_, sizeof(CHDesc) * n, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        count_ = n;
+        tiles_ = tiles;
+        rowBlocks_ = rows;
+        colBlocks_ = cols;
+        dirty_ = false;
+        return hipSuccess;
+    }
+
+    int device_;
+    int step_ = 0;
+    std::vector<CHRequest*> active_;
+    bool dirty_ = true;
+    CHDesc* dDescs_ = nullptr;
+    CHDesc* hDescs_ = nullptr;
+    int cap_ = 0;
+    int count_ = 0, tiles_ = 0, rowBlocks_ = 0, colBlocks_ = 0;
+};
+
+}  // namespace
+}  // namespace custen
+
+using namespace custen;
+
+// ===========================================================================
+// Single simulation
+// ===========================================================================
+struct custenCH {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<CHEngine> engine;
+    std::unique_ptr<BufferPool> pool;
+    CHRequest req;
+};
+
+extern "C" custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0)
+{
+    if (params == nullptr || c0 == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCreate: null argument");
+        return nullptr;
+    }
+    std::unique_ptr<custenCH> ch(new custenCH());
+    std::string why;
+    if (!chSetup(*params, ch->req.co, why)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCreate: " + why);
+        return nullptr;
+    }
+    hipError_t e = hipSetDevice(deviceNum);
+    if (e != hipSuccess) {
+        setHipError(e, "custenCHCreate");
+        return nullptr;
+    }
+    ch->device = deviceNum;
+    if ((e = hipStreamCreate(&ch->stream)) != hipSuccess) {
+        setHipError(e, "custenCHCreate");
+        return nullptr;
+    }
+    ch->engine.reset(new CHEngine(deviceNum));
+    ch->pool.reset(new BufferPool(0));
+    ch->req.params = *params;
+    if ((e = ch->pool->get(params->n, ch->req.buf)) != hipSuccess) {
+        setHipError(e, "custenCHCreate");
+        (void)hipStreamDestroy(ch->stream);
+        return nullptr;
+    }
+    if ((e = ch->engine->admit(&ch->req, c0, ch->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ch->stream)) != hipSuccess) {
+        setHipError(e, "custenCHCreate");
+        ch->pool->put(ch->req.buf);
+        (void)hipStreamDestroy(ch->stream);
+        return nullptr;
+    }
+    ch->req.state = 1;
+    return ch.release();
+}
+
+extern "C" int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream)
+{
+    if (ch == nullptr || steps < 0 || (mode != 0 && mode != 1)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHRun: bad argument");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    (void)hipSetDevice(ch->device);
+    hipStream_t st = stream ? stream : ch->stream;
+    for (int s = 0; s < steps; s++) {
+        hipError_t e = ch->engine->stepAll(st, mode == 1);
+        if (e != hipSuccess) {
+            setHipError(e, "custenCHRun");
+            return lastError();
+        }
+    }
+    ch->req.done += steps;
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" int custenCHGetField(custenCH* ch, double* c, double* cOld)
+{
+    if (ch == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHGetField: null handle");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    (void)hipSetDevice(ch->device);
+    const size_t bytes = (size_t)ch->req.co.n * ch->req.co.n * sizeof(double);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && c)
+        e = hipMemcpy(c, ch->engine->current(&ch->req), bytes, hipMemcpyDefault);
+    if (e == hipSuccess && cOld)
+        e = hipMemcpy(cOld, ch->engine->previous(&ch->req), bytes, hipMemcpyDefault);
+    if (e != hipSuccess) {
+        setHipError(e, "custenCHGetField");
+        return lastError();
+    }
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" void custenCHDestroy(custenCH* ch)
+{
+    if (ch == nullptr) return;
+    (void)hipSetDevice(ch->device);
+    (void)hipStreamSynchronize(ch->stream);
+    ch->engine->remove(&ch->req);
+    ch->pool->put(ch->req.buf);
+    ch->engine.reset();
+    ch->pool.reset();
+    (void)hipStreamDestroy(ch->stream);
+    delete ch;
+}
+
+extern "C" size_t custenCHFootprint(int n) { return n >= 8 ? footprintBytes(n) : 0; }
+
+// ===========================================================================
+// Server
+// ===========================================================================
+struct custenCHServer {
+    int device = 0;
+    size_t budget = 0;
+    int maxSlots = 0;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<CHEngine> engine;
+    std::unique_ptr<BufferPool> pool;
+    std::deque<CHRequest*> queue;
+    std::vector<std::unique_ptr<CHRequest>> all;
+};
+
+extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetBytes, int maxSlots)
+{
+    if (hbmBudgetBytes == 0 || maxSlots < 1) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerCreate: budget and slots must be > 0");
+        return nullptr;
+    }
+    hipError_t e = hipSetDevice(deviceNum);
+    if (e != hipSuccess) {
+        setHipError(e, "custenCHServerCreate");
+        return nullptr;
+    }
+    std::unique_ptr<custenCHServer> s(new custenCHServer());
+    s->device = deviceNum;
+    s->budget = hbmBudgetBytes;
+    s->maxSlots = maxSlots;
+    if ((e = hipStreamCreate(&s->stream)) != hipSuccess) {
+        setHipError(e, "custenCHServerCreate");
+        return nullptr;
+    }
+    s->engine.reset(new CHEngine(deviceNum));
+    s->pool.reset(new BufferPool(hbmBudgetBytes / 4));
+    return s.release();
+}
+
+extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
+                                    const double* c0)
+{
+    if (s == nullptr || params == nullptr || c0 == nullptr || steps < 1) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerSubmit: bad argument");
+        return -1;
+    }
+    std::unique_ptr<CHRequest> r(new CHRequest());
+    std::string why;
+    if (!chSetup(*params, r->co, why)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerSubmit: " + why);
+        return -1;
+    }
+    if (footprintBytes(params->n) > s->budget) {
+        setError(CUSTEN_ERROR_ADMISSION,
+                 "custenCHServerSubmit: request footprint exceeds the HBM budget");
+        return -1;
+    }
+    r->id = (int)s->all.size();
+    r->params = *params;
+    r->steps = steps;
+    const size_t nn = (size_t)params->n * params->n;
+    r->c0.assign(c0, c0 + nn);
+    r->tSubmit = Clock::now();
+    s->queue.push_back(r.get());
+    s->all.push_back(std::move(r));
+    return s->all.back()->id;
+}
+
+namespace {
+
+int serverIteration(custenCHServer* s)
+{
+    (void)hipSetDevice(s->device);
+    hipError_t e;
+    // admission: FIFO while a slot and the HBM budget allow
+    while (!s->queue.empty() && (int)s->engine->active().size() < s->maxSlots) {
+        CHRequest* r = s->queue.front();
+        const size_t fp = footprintBytes(r->co.n);
+        if (s->used + fp > s->budget) break;
+        if ((e = s->pool->get(r->co.n, r->buf)) != hipSuccess) {
+            if (s->engine->active().empty()) {
+                setHipError(e, "custenCHServerStep (allocation)");
+                return -1;
+            }
+            break;  // wait for memory to free up
+        }
+        if ((e = s->engine->admit(r, r->c0.data(), s->stream)) != hipSuccess) {
+            setHipError(e, "custenCHServerStep (admit)");
+            return -1;
+        }
+        s->used += fp;
+        r->state = 1;
+        r->tAdmit = Clock::now();
+        s->queue.pop_front();
+    }
+    if (!s->engine->active().empty()) {
+        if ((e = s->engine->stepAll(s->stream, false)) != hipSuccess ||
+            (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+            setHipError(e, "custenCHServerStep");
+            return -1;
+        }
+        // retire finished requests
+        std::vector<CHRequest*> fin;
+        for (CHRequest* r : s->engine->active()) {
+            r->done++;
+            if (r->done >= r->steps) fin.push_back(r);
+        }
+        for (CHRequest* r : fin) {
+            const size_t nn = (size_t)r->co.n * r->co.n;
+            r->result.resize(nn);
+            e = hipMemcpy(r->result.data(), s->engine->current(r), nn * sizeof(double),
+                          hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                setHipError(e, "custenCHServerStep (result)");
+                return -1;
+            }
+            r->tFinish = Clock::now();
+            r->state = 2;
+            s->engine->remove(r);
+            s->pool->put(r->buf);
+            r->buf = CHBuffers{};
+            s->used -= footprintBytes(r->co.n);
+            std::vector<double>().swap(r->c0);
+        }
+    }
+    return (int)(s->queue.size() + s->engine->active().size());
+}
+
+}  // namespace
+
+extern "C" int custenCHServerStep(custenCHServer* s)
+{
+    if (s == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerStep: null server");
+        return -1;
+    }
+    return serverIteration(s);
+}
+
+extern "C" int custenCHServerRunUntilIdle(custenCHServer* s, int maxIterations)
+{
+    if (s == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerRunUntilIdle: null server");
+        return -1;
+    }
+    int it = 0;
+    while (maxIterations <= 0 || it < maxIterations) {
+        const int left = serverIteration(s);
+        if (left < 0) return -1;
+        it++;
+        if (left == 0) break;
+    }
+    return it;
+}
+
+extern "C" int custenCHServerQuery(custenCHServer* s, int id, int* state, double* latencySubmit,
+                                   double* latencyAdmit)
+{
+    if (s == nullptr || id < 0 || id >= (int)s->all.size()) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerQuery: unknown request");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    const CHRequest* r = s->all[id].get();
+    if (state) *state = r->state;
+    if (r->state == 2) {
+        if (latencySubmit)
+            *latencySubmit = std::chrono::duration<double>(r->tFinish - r->tSubmit).count();
+        if (latencyAdmit)
+            *latencyAdmit = std::chrono::duration<double>(r->tFinish - r->tAdmit).count();
+    } else {
+        if (latencySubmit) *latencySubmit = -1.0;
+        if (latencyAdmit) *latencyAdmit = -1.0;
+    }
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" int custenCHServerResult(custenCHServer* s, int id, double* c)
+{
+    if (s == nullptr || id < 0 || id >= (int)s->all.size() || c == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerResult: bad argument");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    const CHRequest* r = s->all[id].get();
+    if (r->state != 2) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerResult: request not finished");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    std::memcpy(c, r->result.data(), r->result.size() * sizeof(double));
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" void custenCHServerDestroy(custenCHServer* s)
+{
+    if (s == nullptr) return;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    for (CHRequest* r : std::vector<CHRequest*>(s->engine->active())) {
+        s->engine->remove(r);
+        s->pool->put(r->buf);
+    }
+    s->engine.reset();
+    s->pool.reset();
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+}

@@ -1,0 +1,72 @@
+/*
+ * custen_ch.h -- Cahn-Hilliard ADI solver and request server (extension of
+ * the cuSten C ABI; the reference ships this as an application,
+ * cuPentCahnADI/src/cuPentCahnADI.cu, not as a library API).
+ *
+ * Scheme per step (cuPentCahnADI.cu:528-596; serialCahnADI.c:1010-1047):
+ *   cBar = 2c - cOld;  X = L5(cBar) + ((-2/3)(c - cOld) + L3(c^3 - c));
+ *   cyclic pentadiagonal solves of X along x, then along y;
+ *   cOld <- c;  c <- cBar + X.
+ */
+#ifndef CUSTEN_CH_H
+#define CUSTEN_CH_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct custenCH custenCH;
+
+typedef struct {
+    int n;              /* grid n x n, periodic, n >= 8                      */
+    double lx;          /* domain length (reference default 2 pi)            */
+    double D;           /* mobility (1.0)                                    */
+    double gamma;       /* interface parameter (0.01)                        */
+    double dtOverDx;    /* dt = dtOverDx * dx (0.1)                          */
+} custenCHParams;
+
+/* Single-simulation solver with resident fields.  c0: n*n initial condition
+ * (host or device); cOld starts equal to c0 as in the reference. */
+custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0);
+/* Advance `steps` steps.  mode 0: the four step kernels are queued back to
+ * back; mode 1: the same kernels with a stream synchronisation after each
+ * launch (the reference's launch-and-synchronise loop, cuPentCahnADI.cu:528-596),
+ * used as the plain per-grid baseline. */
+int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream);
+/* Copy the current field (n*n doubles) to host memory. */
+int custenCHGetField(custenCH* ch, double* c, double* cOld);
+void custenCHDestroy(custenCH* ch);
+
+/* Request server with continuous batching.  Requests are admitted while their
+ * resident footprint fits in `hbmBudgetBytes` and a slot is free; every engine
+ * iteration advances all resident requests by one step with one batched
+ * launch sequence; finished requests retire and free their slot. */
+typedef struct custenCHServer custenCHServer;
+custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetBytes, int maxSlots);
+/* Queue a request.  Returns its id (>= 0) or -1.  c0 is copied at submit. */
+int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
+                         const double* c0);
+/* Run engine iterations until the queue is empty and all requests finished
+ * (maxIterations <= 0: no limit).  Returns the number of iterations. */
+int custenCHServerRunUntilIdle(custenCHServer* s, int maxIterations);
+/* One engine iteration (admit, step all, retire).  Returns the number of
+ * requests still queued or resident. */
+int custenCHServerStep(custenCHServer* s);
+/* Request state: 0 queued, 1 resident, 2 finished.  On finish, latency in
+ * seconds from submit and from admission are available. */
+int custenCHServerQuery(custenCHServer* s, int id, int* state, double* latencySubmit,
+                        double* latencyAdmit);
+/* Copy a finished request's final field to host memory (n*n doubles). */
+int custenCHServerResult(custenCHServer* s, int id, double* c);
+/* Bytes a request of grid size n keeps resident. */
+size_t custenCHFootprint(int n);
+void custenCHServerDestroy(custenCHServer* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSTEN_CH_H */
