@@ -26,7 +26,8 @@ DEVICE = 0
 HOST = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libcusten.so")
+# CUSTEN_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("CUSTEN_LIBRARY") or os.path.join(_HERE, "lib", "libcusten.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
 HEADER_PATH = os.path.join(INCLUDE_DIR, "cuSten.h")
 

@@ -68,14 +68,21 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, out_dir: str | None = None, defines=()) -> str:
+    """Compile and link.  `defines` (e.g. ["CUSTEN_OCC_F64_LARGE=4"]) and
+    `out_dir` produce tuning variants without touching the default build."""
+    global OBJ, COMMON
+    lib_dir = out_dir or LIBDIR
+    if defines:
+        OBJ = os.path.join(lib_dir, "_obj")
+        COMMON = COMMON + [f"-D{d}" for d in defines]
     os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(lib_dir, exist_ok=True)
     srcs = _sources()
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))
-    so = os.path.join(LIBDIR, "libcusten.so")
+    so = os.path.join(lib_dir, "libcusten.so")
     tmp = so + ".tmp"
     cmd = [HIPCC, "-fgpu-rdc", "--hip-link", f"--offload-arch={ARCH}", "-shared", *objs,
            "-o", tmp, "-lpthread"]
@@ -84,7 +91,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc link failed")
     os.replace(tmp, so)
-    ar = os.path.join(LIBDIR, "libcusten.a")
+    ar = os.path.join(lib_dir, "libcusten.a")
     if os.path.exists(ar):
         os.remove(ar)
     subprocess.run(["ar", "rcs", ar, *objs], check=True)

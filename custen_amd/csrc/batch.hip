@@ -18,6 +18,7 @@
 #include "kernels.h"
 #include "plan_internal.h"
 #include "stencil_body.h"
+#include "tuner.h"
 
 #pragma clang fp contract(off)
 
@@ -107,6 +108,7 @@ struct custenBatch {
     BatchDev* dGrids = nullptr;
     double* dWeights = nullptr;
     int parity = 0;           // 1: the current input is the original output side
+    bool tuned = false;
     hipStream_t own = nullptr;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, start parity)
     std::map<std::pair<int, int>, hipGraph_t> graphDefs;
@@ -151,6 +153,23 @@ hipError_t launchItems(const custenBatch* b, int firstItem, int nItems, int pari
     }
 #undef CUSTEN_BATCH_CASE
     return hipErrorInvalidValue;
+}
+
+// Strip tables for a tile configuration (host copy and device copy).
+hipError_t configure(custenBatch* b, int rpw, int chunks)
+{
+    b->rpw = rpw;
+    b->chunks = chunks;
+    int64_t strips = 0;
+    for (int g = 0; g < b->numGrids; g++) {
+        b->host[g].stripBase = (int)strips;
+        b->strips[g] = stripsFor(b->host[g].nx, b->host[g].ny, rpw, chunks);
+        strips += b->strips[g];
+    }
+    if (strips > (int64_t)0x7fffffff) return hipErrorInvalidValue;
+    b->totalStrips = (int)strips;
+    return hipMemcpy(b->dGrids, b->host.data(), sizeof(BatchDev) * b->numGrids,
+                     hipMemcpyHostToDevice);
 }
 
 // `steps` applies starting at parity p0, all grids in one launch per step.
@@ -261,6 +280,47 @@ extern "C" int custenBatchRun(custenBatch* b, int steps, int mode, hipStream_t s
                 e = launchItems(b, b->host[g].stripBase, b->strips[g], par, st);
         }
     } else if (steps > 0) {
+        if (!b->tuned && tunerEnabled()) {
+            // trial applies write the output side without advancing the
+            // state (the next real step overwrites it with the same values)
+            TuneKey k;
+            k.kind = 1;
+            k.dtype = static_cast<int>(b->dtype);
+            k.H = b->H;
+            k.V = b->V;
+            int64_t pts = 0;
+            int maxSide = 0;
+            for (const BatchDev& d : b->host) {
+                pts += (int64_t)d.nx * d.ny;
+                maxSide = std::max(maxSide, std::max(d.nx, d.ny));
+            }
+            k.nx = (int)(pts >> 20);
+            k.ny = maxSide;
+            k.count = b->numGrids;
+            TuneVal v;
+            if (!tunerLookup(k, v)) {
+                static const std::vector<TuneVal> cands = {{4, 1}, {4, 4}, {8, 1}, {8, 2},
+                                                           {8, 4}, {8, 8}};
+                auto run = [&](const TuneVal& c) {
+                    hipError_t err = hipSuccess;
+                    if (c.rpw != b->rpw || c.chunks != b->chunks) {
+                        err = hipStreamSynchronize(b->own);
+                        if (err == hipSuccess) err = configure(b, c.rpw, c.chunks);
+                    }
+                    if (err != hipSuccess) return err;
+                    return launchItems(b, 0, b->totalStrips, b->parity, b->own);
+                };
+                e = tunerMeasure(b->own, run, cands, 2, v);
+                if (e == hipSuccess) tunerStore(k, v);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(b->own);
+            if (e == hipSuccess) e = configure(b, v.rpw, v.chunks);
+            if (e != hipSuccess) {
+                setHipError(e, fn);
+                return lastError();
+            }
+            b->tuned = true;
+        }
         const auto key = std::make_pair(steps, b->parity);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {

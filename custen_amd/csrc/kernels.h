@@ -53,6 +53,10 @@ struct StencilLaunch {
     int rowsPerWave;         // 0 = default
     int chunks;              // row chunks per workgroup, 0 = default
     int vecIn, vecOut;       // set by launchStencil: rows are pair-aligned
+    // column strips handled by one launch: interior launches cover
+    // tx = stripTx0 .. stripTx0 + nStripTx - 1, edge launches the listed tx
+    int stripTx0, nStripTx;
+    int edgeTx[4];
 };
 
 // Point-function apply.  Same geometry as StencilLaunch; `func` is a device

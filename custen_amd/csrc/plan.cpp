@@ -25,9 +25,11 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "kernels.h"
 #include "plan_internal.h"
+#include "tuner.h"
 
 namespace custen {
 
@@ -408,8 +410,8 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
         s.yGlobal0 = r.yGlobal0;
         s.zeroRight = p->zeroRight ? 1 : 0;
         s.xform = Xform::Identity;
-        s.rowsPerWave = p->rowsPerWave;
-        s.chunks = p->chunks;
+        s.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
+        s.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
         return launchStencil(s, st);
     }
     FunLaunch f{};
@@ -435,8 +437,8 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
     f.wy0 = wy0;
     f.wy1 = wy1;
     f.yGlobal0 = r.yGlobal0;
-    f.rowsPerWave = p->rowsPerWave;
-    f.chunks = p->chunks;
+    f.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
+    f.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
     return launchFun(f, st);
 }
 
@@ -630,6 +632,40 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             r.rows.rowMax = h->ny - 1;
         }
         r.yGlobal0 = 0;
+        // First resident Compute of a large grid: pick the tile configuration
+        // (bit-identical results for every candidate; synchronous once).
+        if (!p->tuned && p->rowsPerWave == 0 && p->chunks == 0 && tunerEnabled() &&
+            (int64_t)h->nx * h->ny >= (int64_t(1) << 20) && !streamCapturing(st)) {
+            TuneKey k;
+            k.kind = 0;
+            k.dtype = static_cast<int>(p->dtype);
+            k.H = p->H;
+            k.V = p->V;
+            k.xform = p->fun ? 100 + static_cast<int>(p->forceIndirect ? Builtin::None : p->builtin)
+                             : 0;
+            k.nx = h->nx;
+            k.ny = h->ny;
+            TuneVal v;
+            if (!tunerLookup(k, v)) {
+                static const std::vector<TuneVal> cands = {{4, 1}, {4, 2}, {4, 4}, {8, 1},
+                                                           {8, 2}, {8, 4}, {8, 8}, {8, 16}};
+                auto run = [&](const TuneVal& c) {
+                    Plan tmp = *p;
+                    tmp.rowsPerWave = c.rpw;
+                    tmp.chunks = c.chunks;
+                    return launchRegion(h, &tmp, r, params, st);
+                };
+                e = tunerMeasure(st, run, cands, 3, v);
+                if (e != hipSuccess) {
+                    setHipError(e, fn);
+                    return;
+                }
+                tunerStore(k, v);
+            }
+            p->tuned = true;
+            p->tunedRpw = v.rpw;
+            p->tunedChunks = v.chunks;
+        }
         e = launchRegion(h, p, r, params, st);
         if (e != hipSuccess) setHipError(e, fn);
         return;
@@ -964,6 +1000,7 @@ void custenSetTileConfig(cuSten_t* h, int rowsPerWave, int chunksPerGroup)
     }
     p->rowsPerWave = rowsPerWave;
     p->chunks = chunksPerGroup;
+    if (rowsPerWave == 0 && chunksPerGroup == 0) p->tuned = false;  // re-tune on next Compute
     h->mem_shared = stencilLdsBytes(p->dtype, p->H, p->V, rowsPerWave);
 }
 

@@ -28,12 +28,12 @@ struct Plan {
     Builtin builtin = Builtin::None;
     bool forceIndirect = false;
     bool zeroRight = false;     // x-np: right R columns written as 0.0
-    // float-storage handles keep their float pointers here (the double**
-    // tables in cuSten_t hold the same addresses)
     hipStream_t userStream = nullptr;
     bool hasUserStream = false;
-    int rowsPerWave = 0;
+    int rowsPerWave = 0;        // set by custenSetTileConfig (0: autotuned)
     int chunks = 0;
+    bool tuned = false;         // autotuner result for resident launches
+    int tunedRpw = 0, tunedChunks = 0;
     // device copy of host-resident weights / coefficients
     double* paramDev = nullptr;
     size_t paramCap = 0;

@@ -10,12 +10,12 @@
 namespace custen {
 namespace dev {
 
-// Process column strip `g` of the region described by `a`: all of its row
-// chunks.  `w` holds the H*V weights, `tile` the LDS staging buffer of
+// Process strip (tx, tyg) of the region described by `a`: column strip tx,
+// row chunks [tyg * chunks, (tyg + 1) * chunks).  `w` holds the H*V weights, `tile` the LDS staging buffer of
 // (4*RPW + V - 1) x ldsRowLength(H) elements.
-template <typename T, int H, int V, int RPW, Xform X>
-__device__ __forceinline__ void stencilStrip(const StencilLaunch& a, const T (&w)[H * V],
-                                             T* __restrict__ tile, int g)
+template <typename T, int H, int V, int RPW, Xform X, bool FAST>
+__device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T (&w)[H * V],
+                                                 T* __restrict__ tile, int tx, int tyg)
 {
     constexpr int TH = kWaves * RPW;
     constexpr int NR = TH + V - 1;
@@ -25,26 +25,31 @@ __device__ __forceinline__ void stencilStrip(const StencilLaunch& a, const T (&w
     T* __restrict__ out = static_cast<T*>(a.out);
 
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     const int L = a.L;
     const int R = H - 1 - L;
     const int LP = (L + 1) & ~1;
     const int nx = a.nx;
-    const int tilesX = (nx + kTW - 1) / kTW;
-    const int tx = g % tilesX;
-    const int tyg = g / tilesX;
     const int x0 = tx * kTW;
     const bool inside = x0 + kTW <= nx;
     const bool vecIn = a.vecIn && inside;
-    const bool fullRow = a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1;
+    const bool fullRow = FAST || (a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1);
+
+    const int y00 = tyg * a.chunks * TH;
+    if (y00 >= a.ny) return;
+    StagedRows<T, NR> st;
+    loadRows<T, NR, FAST>(st, in, a.ldIn, a.rows, y00, a.T, x0, nx, L, R, a.xWrap, vecIn, wave, lane);
 
     for (int c = 0; c < a.chunks; c++) {
-        const int y0 = (tyg * a.chunks + c) * TH;
-        if (y0 >= a.ny) break;
-        if (c > 0) __syncthreads();
-        stageTile<T, X>(tile, LD, NR, in, a.ldIn, a.rows, y0, a.T, x0, nx, L, R, LP, a.xWrap,
-                        vecIn, wave, lane);
+        const int y0 = y00 + c * TH;
+        if (c > 0) __syncthreads();  // the previous chunk's compute is done with LDS
+        storeRows<T, X, NR>(st, tile, LD, L, R, LP, wave, lane);
         __syncthreads();
+        const bool more = (c + 1 < a.chunks) && (y0 + TH < a.ny);
+        // prefetch the next chunk's rows; they stay in flight during the compute
+        if (more)
+            loadRows<T, NR, FAST>(st, in, a.ldIn, a.rows, y0 + TH, a.T, x0, nx, L, R, a.xWrap, vecIn,
+                            wave, lane);
 
         T acc[RPW][2];
 #pragma unroll
@@ -78,7 +83,40 @@ __device__ __forceinline__ void stencilStrip(const StencilLaunch& a, const T (&w
             storePair<T>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r][0], acc[r][1], fullRow,
                          nx, a.wx0, a.wx1, a.zeroRight, R);
         }
+        if (!more) break;
     }
+}
+
+// True when column strip tx and its column halo lie inside [0, nx), its rows
+// are pair-aligned and fully written, and one wrap suffices for row indices:
+// such strips run the FAST instantiation.
+__host__ __device__ inline bool stripIsInterior(const StencilLaunch& a, int tx)
+{
+    const int x0 = tx * kTW;
+    const int R = a.H - 1 - a.L;
+    return a.vecIn && a.vecOut && x0 - a.L >= 0 && x0 + kTW + R <= a.nx && x0 >= a.wx0 &&
+           x0 + kTW <= a.wx1 && (a.rows.wrap ? a.rows.rowWrap >= a.V : true);
+}
+
+// Strip g of a single-grid launch -> (tx, tyg).
+__device__ __forceinline__ void stripOf(const StencilLaunch& a, bool interior, int g, int& tx,
+                                        int& tyg)
+{
+    const int k = g % a.nStripTx;
+    tyg = g / a.nStripTx;
+    tx = interior ? a.stripTx0 + k : a.edgeTx[k];
+}
+
+// Batched grids: one strip id covers all columns of a grid; both paths live in
+// the same kernel.
+template <typename T, int H, int V, int RPW, Xform X>
+__device__ __forceinline__ void stencilStrip(const StencilLaunch& a, const T (&w)[H * V],
+                                             T* __restrict__ tile, int g)
+{
+    const int tilesX = (a.nx + kTW - 1) / kTW;
+    const int tx = g % tilesX, tyg = g / tilesX;
+    if (stripIsInterior(a, tx)) stencilStripImpl<T, H, V, RPW, X, true>(a, w, tile, tx, tyg);
+    else stencilStripImpl<T, H, V, RPW, X, false>(a, w, tile, tx, tyg);
 }
 
 }  // namespace dev

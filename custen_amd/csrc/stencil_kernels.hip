@@ -27,14 +27,29 @@ namespace dev {
 // ---------------------------------------------------------------------------
 // Fixed-shape kernel
 // ---------------------------------------------------------------------------
-template <typename T, int H, int V, int RPW, Xform X>
-__global__ __launch_bounds__(kThreads) void stencilFixed(StencilLaunch a)
+// Occupancy target per shape: the register-blocked window of the larger
+// stencils is kept within 3 waves per SIMD without spilling.
+#ifndef CUSTEN_OCC_F64_LARGE
+#define CUSTEN_OCC_F64_LARGE 3
+#endif
+template <typename T, int H, int V, int RPW>
+struct FixedOccupancy {
+    static constexpr int value =
+        (sizeof(T) == 8 && H * V >= 25 && RPW == 4) ? CUSTEN_OCC_F64_LARGE : 1;
+};
+
+template <typename T, int H, int V, int RPW, Xform X, bool FAST>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(FixedOccupancy<T, H, V, RPW>::value, 8))) void stencilFixed(
+    StencilLaunch a)
 {
     __shared__ __attribute__((aligned(16))) T tile[(kWaves * RPW + V - 1) * ldsRowLength(H)];
     T w[H * V];
 #pragma unroll
     for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
-    stencilStrip<T, H, V, RPW, X>(a, w, tile, xcdRemap(blockIdx.x, gridDim.x));
+    int tx, tyg;
+    stripOf(a, FAST, xcdRemap(blockIdx.x, gridDim.x), tx, tyg);
+    stencilStripImpl<T, H, V, RPW, X, FAST>(a, w, tile, tx, tyg);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,13 +320,45 @@ int gridSize(int nx, int ny, int TH, int chunks)
     return tilesX * tilesY;
 }
 
+// Interior column strips run the FAST kernel, the (at most four) edge strips
+// the general one.
 template <typename T, int H, int V, int RPW, Xform X>
-hipError_t runFixed(const StencilLaunch& s, hipStream_t st)
+hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
 {
-    const int grid = gridSize(s.nx, s.ny, dev::kWaves * RPW, s.chunks);
-    hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X>), dim3(grid), dim3(dev::kThreads), 0,
-                       st, s);
-    return hipGetLastError();
+    StencilLaunch s = in;
+    const int tilesX = (s.nx + dev::kTW - 1) / dev::kTW;
+    const int rowsPerGroup = dev::kWaves * RPW * s.chunks;
+    const int tilesY = (s.ny + rowsPerGroup - 1) / rowsPerGroup;
+    int lo = 0, hi = tilesX;  // interior range [lo, hi)
+    while (lo < hi && !dev::stripIsInterior(s, lo)) lo++;
+    while (hi > lo && !dev::stripIsInterior(s, hi - 1)) hi--;
+    for (int t = lo; t < hi; t++)
+        if (!dev::stripIsInterior(s, t)) return hipErrorInvalidValue;  // not contiguous
+    StencilLaunch e = s;
+    e.nStripTx = 0;
+    for (int t = 0; t < tilesX; t++) {
+        if (t >= lo && t < hi) continue;
+        if (e.nStripTx == 4) return hipErrorInvalidValue;
+        e.edgeTx[e.nStripTx++] = t;
+    }
+    // edge strips first, one row chunk per workgroup: they are a few percent
+    // of the work and must not leave a tail after the interior launch
+    if (e.nStripTx > 0) {
+        e.chunks = 1;
+        const int tilesYe = (s.ny + dev::kWaves * RPW - 1) / (dev::kWaves * RPW);
+        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, false>),
+                           dim3(e.nStripTx * tilesYe), dim3(dev::kThreads), 0, st, e);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
+    if (hi > lo) {
+        s.stripTx0 = lo;
+        s.nStripTx = hi - lo;
+        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, true>),
+                           dim3(s.nStripTx * tilesY), dim3(dev::kThreads), 0, st, s);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 template <typename T, Xform X>
@@ -364,7 +411,8 @@ hipError_t launchStencil(const StencilLaunch& in, hipStream_t st)
     StencilLaunch s = in;
     if (s.nx <= 0 || s.ny <= 0) return hipSuccess;
     if (s.H < 1 || s.V < 1 || s.H > 65 || s.V > 65) return hipErrorInvalidValue;
-    if (s.rowsPerWave != 4 && s.rowsPerWave != 8) s.rowsPerWave = kDefaultRPW;
+    if (s.rowsPerWave != 4 && s.rowsPerWave != 8)
+        s.rowsPerWave = (s.dtype == Dtype::F64 && s.H * s.V >= 25) ? 4 : kDefaultRPW;
     if (s.chunks <= 0) s.chunks = kDefaultChunks;
     if (s.dtype == Dtype::F64) {
         s.vecIn = pairAligned<double>(s.in, s.ldIn);

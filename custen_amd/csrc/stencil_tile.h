@@ -127,6 +127,81 @@ __device__ __forceinline__ void stageTile(T* __restrict__ tile, int LD, int nrow
     }
 }
 
+// Rows of one chunk held in registers between their global load and their
+// LDS store, so that a wave issues all its row loads before it waits on any
+// of them (and, with prefetch, while the previous chunk is being computed).
+template <typename T, int NR>
+struct StagedRows {
+    static constexpr int PER = (NR + kWaves - 1) / kWaves;  // rows per wave
+    typename Vec2<T>::type body[PER];
+    T halo[PER];
+};
+
+// Source row of region row gy when the region is at least as tall as the
+// window (one correction at most); the general mapRow otherwise.
+__device__ __forceinline__ int mapRowNear(const RowMap& m, int gy)
+{
+    if (m.wrap) {
+        if (gy < 0) gy += m.rowWrap;
+        else if (gy >= m.rowWrap) gy -= m.rowWrap;
+        return gy;
+    }
+    return gy < m.rowMin ? m.rowMin : (gy > m.rowMax ? m.rowMax : gy);
+}
+
+// FAST: the strip and its column halo lie inside [0, nx) with pair-aligned
+// rows, and a row index needs at most one wrap (ny >= window height): no
+// column mapping and no division in the address arithmetic.
+template <typename T, int NR, bool FAST>
+__device__ __forceinline__ void loadRows(StagedRows<T, NR>& s, const T* __restrict__ in,
+                                         int64_t ldIn, const RowMap& rows, int y0, int T_, int x0,
+                                         int nx, int L, int R, int xWrap, bool vecIn, int wave,
+                                         int lane)
+{
+    using V2 = typename Vec2<T>::type;
+#pragma unroll
+    for (int i = 0; i < StagedRows<T, NR>::PER; i++) {
+        const int r = wave + i * kWaves;
+        if (r < NR) {
+            const int sy = FAST ? mapRowNear(rows, y0 - T_ + r) : mapRow(rows, y0 - T_ + r);
+            const T* grow = in + (int64_t)sy * ldIn;
+            const int gx = x0 + 2 * lane;
+            if (FAST || vecIn) {
+                s.body[i] = *reinterpret_cast<const V2*>(grow + gx);
+            } else {
+                s.body[i].x = grow[mapCol(gx, nx, xWrap)];
+                s.body[i].y = grow[mapCol(gx + 1, nx, xWrap)];
+            }
+            if (lane < L + R) {
+                const int hx = lane < L ? x0 - L + lane : x0 + kTW + (lane - L);
+                s.halo[i] = FAST ? grow[hx] : grow[mapCol(hx, nx, xWrap)];
+            }
+        }
+    }
+}
+
+template <typename T, Xform X, int NR>
+__device__ __forceinline__ void storeRows(const StagedRows<T, NR>& s, T* __restrict__ tile, int LD,
+                                          int L, int R, int LP, int wave, int lane)
+{
+    using V2 = typename Vec2<T>::type;
+#pragma unroll
+    for (int i = 0; i < StagedRows<T, NR>::PER; i++) {
+        const int r = wave + i * kWaves;
+        if (r < NR) {
+            T* lrow = tile + r * LD;
+            V2 o;
+            o.x = xformValue<T, X>(s.body[i].x);
+            o.y = xformValue<T, X>(s.body[i].y);
+            *reinterpret_cast<V2*>(lrow + LP + 2 * lane) = o;
+            if (lane < L + R) {
+                const int hx = lane < L ? LP - L + lane : LP + kTW + (lane - L);
+                lrow[hx] = xformValue<T, X>(s.halo[i]);
+            }
+        }
+    }
+}
+
 // Write one output row's two values for this lane, honouring the write window.
 template <typename T>
 __device__ __forceinline__ void storePair(T* __restrict__ orow, int gx, T v0, T v1, bool fullRow,

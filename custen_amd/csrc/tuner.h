@@ -1,0 +1,35 @@
+// tuner.h -- tile-configuration autotuner (see tuner.cpp).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <functional>
+#include <vector>
+
+namespace custen {
+
+struct TuneKey {
+    int kind = 0;   // 0 single grid, 1 batched grids
+    int dtype = 0;
+    int H = 0, V = 0;
+    int xform = 0;
+    int nx = 0, ny = 0;
+    int count = 0;  // batched: number of grids (nx, ny: total points / 2^20, max grid side)
+    bool operator<(const TuneKey& o) const;
+};
+
+struct TuneVal {
+    int rpw = 8;
+    int chunks = 4;
+};
+
+bool tunerEnabled();
+bool tunerLookup(const TuneKey& k, TuneVal& v);
+void tunerStore(const TuneKey& k, const TuneVal& v);
+// Runs every candidate once untimed and `reps` times timed on `st`; the
+// fastest lands in `best`.
+hipError_t tunerMeasure(hipStream_t st, const std::function<hipError_t(const TuneVal&)>& run,
+                        const std::vector<TuneVal>& cands, int reps, TuneVal& best);
+bool streamCapturing(hipStream_t st);
+
+}  // namespace custen

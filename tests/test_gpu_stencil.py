@@ -34,7 +34,7 @@ def create_weights(variant, h, nt, nx, ny, out, inp, w, H, L, V, T, bx=32, by=8)
 
 
 def oracle_weights(variant, u, w, H, L, V, T, init):
-    periodic = variant.endswith("p")
+    periodic = not variant.endswith("np")
     out = init.copy()
     if variant.startswith("2DXY"):
         return oracle.apply(u, w, H, L, V, T, periodic, out=out)
@@ -89,7 +89,7 @@ def test_weights_device(gpu, variant, nx, ny, nt, H, L, V, T):
     assert mismatches(got, ref) == 0, f"{mismatches(got, ref)} points differ"
     hh, ll, vv, tt = (H, L, 1, 0) if variant.startswith("2DX") and not variant.startswith("2DXY") \
         else ((1, 0, V, T) if variant.startswith("2DY") else (H, L, V, T))
-    assert scaled_error(got, ref, u, w, hh, ll, vv, tt, variant.endswith("p")) <= 1e-14
+    assert scaled_error(got, ref, u, w, hh, ll, vv, tt, not variant.endswith("np")) <= 1e-14
 
 
 @pytest.mark.parametrize("variant", ["2DXp", "2DXnp", "2DYp", "2DYnp", "2DXYp", "2DXYnp"])
