@@ -290,9 +290,20 @@ def bench_ch(args, torch, api, extra):
     t_plain = time.perf_counter() - t0
     plain_rate = count / t_plain
     log(f"ch plain: {count} requests in {t_plain:.2f} s")
+    # one request at a time on the fused kernels (no batching)
+    t0 = time.perf_counter()
+    for r, (_, n, _) in enumerate(reqs):
+        s = api.CHSolver(api.ch_params(n), ics[r])
+        s.run(steps, mode=0)
+        s.field()
+        s.close()
+    t_single = time.perf_counter() - t0
 
     out = {"requests": count, "steps": steps, "sizes": sorted(set(n for _, n, _ in reqs)),
-           "plain_solves_per_s": round(plain_rate, 2), "plain_s": round(t_plain, 3)}
+           "plain_solves_per_s": round(plain_rate, 2), "plain_s": round(t_plain, 3),
+           "plain": "reference-shaped launch loop: 13 launches + synchronisations per step, "
+                    "one request at a time",
+           "fused_one_at_a_time_solves_per_s": round(count / t_single, 2)}
     budget = int(args.ch_budget_gb * (1 << 30))
     for load in (0.5, 2.0):
         arrivals = workloads.ch_requests(count, rate_per_s=load * plain_rate)

@@ -24,6 +24,7 @@
 #include "../../include/cuSten.h"
 #include "../../include/custen_ch.h"
 #include "ch_kernels.h"
+#include "ch_plain.h"
 #include "plan_internal.h"
 
 #pragma STDC FP_CONTRACT OFF
@@ -156,13 +157,14 @@ struct CHBuffers {
     int n = 0;
     double* f[2] = {nullptr, nullptr};
     double* X = nullptr;
+    double* Y = nullptr;
     double* fac = nullptr;
 };
 
 size_t footprintBytes(int n)
 {
     const size_t nn = (size_t)n * n;
-    return 3 * nn * sizeof(double) + (size_t)7 * (n - 2) * sizeof(double);
+    return 4 * nn * sizeof(double) + (size_t)7 * (n - 2) * sizeof(double);
 }
 
 class BufferPool {
@@ -189,6 +191,7 @@ public:
         if ((e = hipMalloc(&out.f[0], nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.f[1], nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.X, nn * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc(&out.Y, nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.fac, (size_t)7 * (n - 2) * sizeof(double))) != hipSuccess) {
             release(out);
             return e;
@@ -209,9 +212,9 @@ public:
 private:
     static void release(CHBuffers& b)
     {
-        for (double* p : {b.f[0], b.f[1], b.X, b.fac})
+        for (double* p : {b.f[0], b.f[1], b.X, b.Y, b.fac})
             if (p) (void)hipFree(p);
-        b.f[0] = b.f[1] = b.X = b.fac = nullptr;
+        b.f[0] = b.f[1] = b.X = b.Y = b.fac = nullptr;
     }
     size_t keep_;
     size_t held_ = 0;
@@ -277,8 +280,17 @@ public:
         return r->buf.f[((step_ - r->admitStep) & 1) ^ 1];
     }
 
+    // A step done outside the engine (the plain path) that kept the field in
+    // place: advance the step counter without flipping the request's roles.
+    void noteExternalStep(CHRequest* r)
+    {
+        step_++;
+        r->admitStep++;
+        dirty_ = true;
+    }
+
     // One step of all resident requests.  plain = true synchronises the stream
-    // after every launch (the reference's launch-and-synchronise pattern).
+    // after every launch.
     hipError_t stepAll(hipStream_t st, bool plain)
     {
         if (active_.empty()) return hipSuccess;
@@ -287,9 +299,13 @@ public:
         auto sync = [&]() { return plain ? hipStreamSynchronize(st) : hipSuccess; };
         if ((e = chLaunchRhs(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchXSolve(dDescs_, count_, rowBlocks_, st)) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 0, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchYSolve(dDescs_, count_, colBlocks_, st)) != hipSuccess) return e;
+        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 1, st)) != hipSuccess) return e;
+        if ((e = sync()) != hipSuccess) return e;
+        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
         if ((e = chLaunchFinal(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
@@ -317,7 +333,7 @@ private:
             hipError_t e = hipStreamSynchronize(st);
             if (e != hipSuccess) return e;
         }
-        int tiles = 0, rows = 0, cols = 0;
+        int tiles = 0, trs = 0, cols = 0;
         for (int i = 0; i < n; i++) {
             const CHRequest* r = active_[i];
             CHDesc& d = hDescs_[i];
@@ -325,6 +341,7 @@ private:
             d.f[0] = r->buf.f[0];
             d.f[1] = r->buf.f[1];
             d.X = r->buf.X;
+            d.Y = r->buf.Y;
             d.fac = r->buf.fac;
             d.a = r->co.a;
             d.b = r->co.b;
@@ -336,17 +353,17 @@ private:
             d.n = r->co.n;
             d.admitStep = r->admitStep;
             d.tileBase = tiles;
-            d.rowBase = rows;
             d.colBase = cols;
+            d.trBase = trs;
             tiles += chTiles(d.n);
-            rows += chRowBlocks(d.n);
+            trs += chTrTiles(d.n);
             cols += chColBlocks(d.n);
         }
         hipError_t e = hipMemcpyAsync(dDescs_, hDescs_, sizeof(CHDesc) * n, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
         count_ = n;
         tiles_ = tiles;
-        rowBlocks_ = rows;
+        trTiles_ = trs;
         colBlocks_ = cols;
         dirty_ = false;
         return hipSuccess;
@@ -359,7 +376,7 @@ private:
     CHDesc* dDescs_ = nullptr;
     CHDesc* hDescs_ = nullptr;
     int cap_ = 0;
-    int count_ = 0, tiles_ = 0, rowBlocks_ = 0, colBlocks_ = 0;
+    int count_ = 0, tiles_ = 0, trTiles_ = 0, colBlocks_ = 0;
 };
 
 }  // namespace
@@ -376,7 +393,62 @@ struct custenCH {
     std::unique_ptr<CHEngine> engine;
     std::unique_ptr<BufferPool> pool;
     CHRequest req;
+    // reference-shaped baseline (allocated on first use)
+    PlainCH plain;
+    double* wDev = nullptr;  // 25 linear + 9 nonlinear weights
+    cuSten_t lin{}, nl{};
+    bool linReady = false, nlReady = false;
+    const double* nlIn = nullptr;
 };
+
+namespace {
+
+void releasePlain(custenCH* ch)
+{
+    if (ch->linReady) cuStenDestroy2DXYp(&ch->lin);
+    if (ch->nlReady) cuStenDestroy2DXYpFun(&ch->nl);
+    ch->linReady = ch->nlReady = false;
+    for (double* p : {ch->plain.cBar, ch->plain.nonLin, ch->plain.cHalf, ch->plain.multi, ch->wDev})
+        if (p) (void)hipFree(p);
+    ch->plain.cBar = ch->plain.nonLin = ch->plain.cHalf = ch->plain.multi = ch->wDev = nullptr;
+}
+
+hipError_t preparePlain(custenCH* ch)
+{
+    if (ch->plain.cBar) return hipSuccess;
+    const CHCoeffs& co = ch->req.co;
+    const int n = co.n, m = n - 2;
+    const size_t nn = (size_t)n * n;
+    PlainCH& p = ch->plain;
+    p.n = n;
+    p.a = co.a;
+    p.b = co.b;
+    p.d = co.d;
+    p.e = co.e;
+    for (int k = 0; k < 4; k++) p.omega[k] = co.omega[k];
+    hipError_t e;
+    if ((e = hipMalloc(&p.cBar, nn * 8)) != hipSuccess ||
+        (e = hipMalloc(&p.nonLin, nn * 8)) != hipSuccess ||
+        (e = hipMalloc(&p.cHalf, nn * 8)) != hipSuccess ||
+        (e = hipMalloc(&p.multi, (size_t)7 * m * n * 8)) != hipSuccess ||
+        (e = hipMalloc(&ch->wDev, 34 * 8)) != hipSuccess)
+        return e;
+    // replicate every factored diagonal and inverse across the n systems
+    // (setMultiLHS + pentFactorBatch and the inv1Multi/inv2Multi loop of
+    // cuPentCahnADI.cu:382-446 give these same values)
+    std::vector<double> rep((size_t)7 * m * n);
+    for (int a = 0; a < 7; a++)
+        for (int i = 0; i < m; i++)
+            std::fill_n(rep.begin() + ((size_t)a * m + i) * n, n, co.fac[(size_t)a * m + i]);
+    if ((e = hipMemcpy(p.multi, rep.data(), rep.size() * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    double w[34];
+    std::memcpy(w, co.wlin, sizeof(co.wlin));
+    std::memcpy(w + 25, co.wnl, sizeof(co.wnl));
+    return hipMemcpy(ch->wDev, w, sizeof(w), hipMemcpyHostToDevice);
+}
+
+}  // namespace
 
 extern "C" custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0)
 {
@@ -427,12 +499,59 @@ extern "C" int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream
     }
     (void)hipSetDevice(ch->device);
     hipStream_t st = stream ? stream : ch->stream;
+    if (mode == 0) {
+        for (int s = 0; s < steps; s++) {
+            hipError_t e = ch->engine->stepAll(st, false);
+            if (e != hipSuccess) {
+                setHipError(e, "custenCHRun");
+                return lastError();
+            }
+        }
+        ch->req.done += steps;
+        return CUSTEN_SUCCESS;
+    }
+    // mode 1: reference-shaped launch-and-synchronise loop
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = preparePlain(ch);
+    if (e != hipSuccess) {
+        setHipError(e, "custenCHRun");
+        return lastError();
+    }
+    const int n = ch->req.co.n;
+    auto stencil = [&](double* out, const double* in, bool nonlinear) -> hipError_t {
+        if (nonlinear) {
+            if (!ch->nlReady || ch->nlIn != in) {
+                if (ch->nlReady) cuStenDestroy2DXYpFun(&ch->nl);
+                double* f = custenGetBuiltinFunction("xy_cube_minus", ch->device);
+                cuStenCreate2DXYpFun(&ch->nl, ch->device, 1, n, n, 8, 8, out,
+                                     const_cast<double*>(in), ch->wDev + 25, 3, 1, 1, 3, 1, 1, f);
+                if (lastError() != CUSTEN_SUCCESS) return hipErrorInvalidValue;
+                custenSetStream(&ch->nl, st);
+                ch->nlReady = true;
+                ch->nlIn = in;
+            }
+            cuStenCompute2DXYpFun(&ch->nl, DEVICE);
+        } else {
+            if (!ch->linReady) {
+                cuStenCreate2DXYp(&ch->lin, ch->device, 1, n, n, 32, 32, out,
+                                  const_cast<double*>(in), ch->wDev, 5, 2, 2, 5, 2, 2);
+                if (lastError() != CUSTEN_SUCCESS) return hipErrorInvalidValue;
+                custenSetStream(&ch->lin, st);
+                ch->linReady = true;
+            }
+            cuStenCompute2DXYp(&ch->lin, DEVICE);
+        }
+        return lastError() == CUSTEN_SUCCESS ? hipSuccess : hipErrorInvalidValue;
+    };
     for (int s = 0; s < steps; s++) {
-        hipError_t e = ch->engine->stepAll(st, mode == 1);
+        double* cur = ch->engine->current(&ch->req);
+        double* old = ch->engine->previous(&ch->req);
+        e = chPlainStep(ch->plain, cur, old, st, stencil);
         if (e != hipSuccess) {
-            setHipError(e, "custenCHRun");
+            if (lastError() == CUSTEN_SUCCESS) setHipError(e, "custenCHRun");
             return lastError();
         }
+        ch->engine->noteExternalStep(&ch->req);
     }
     ch->req.done += steps;
     return CUSTEN_SUCCESS;
@@ -463,6 +582,7 @@ extern "C" void custenCHDestroy(custenCH* ch)
     if (ch == nullptr) return;
     (void)hipSetDevice(ch->device);
     (void)hipStreamSynchronize(ch->stream);
+    releasePlain(ch);
     ch->engine->remove(&ch->req);
     ch->pool->put(ch->req.buf);
     ch->engine.reset();

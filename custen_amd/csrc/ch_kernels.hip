@@ -5,13 +5,13 @@
 // serial form cuPentSpeedUp/serialCahnADITiming/serialCahnADI.c:1010-1047)
 // in four phases:
 //   rhs   : cBar = 2c - cOld, X = L5x5(cBar) + ((-2/3)(c - cOld) + L3x3(c^3 - c))
-//   xsolve: cyclic pentadiagonal solve of every row of X (rows staged in LDS,
-//           pentSolve + solveEnd + solveFull in place)
-//   ysolve: cyclic solve of every column (thread per column; pentSolve +
-//           solveEnd; the column solveFull is applied in `final`)
+//   xsolve: Y = X^T, cyclic pentadiagonal solve of every column of Y
+//           (pentSolve + solveEnd), X = Y^T with solveFull applied on the way
+//   ysolve: cyclic solve of every column of X (pentSolve + solveEnd)
 //   final : c_new = cBar + (X - (inv1 x~2 + inv2 x~1)), written over cOld
-// The reference's two cublasDgeam transposes are not needed: rows are solved
-// through LDS and columns with coalesced thread-per-column sweeps.
+// Every solve is a thread-per-column sweep (coalesced across the wave) with a
+// block of row loads in flight ahead of the recurrence; the transposes
+// replace the reference's cublasDgeam calls (cuPentCahnADI.cu:552, 566).
 // Compiled with -ffp-contract=off: every operation is rounded as written, the
 // stencil sums are explicit fma chains (the cuSten order).
 #include <hip/hip_runtime.h>
@@ -29,7 +29,7 @@ __device__ __forceinline__ int findRequest(const CHDesc* d, int count, int item,
     int lo = 0, hi = count - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int b = which == 0 ? d[mid].tileBase : (which == 1 ? d[mid].rowBase : d[mid].colBase);
+        const int b = which == 0 ? d[mid].tileBase : (which == 2 ? d[mid].colBase : d[mid].trBase);
         if (b <= item) lo = mid;
         else hi = mid - 1;
     }
@@ -112,77 +112,61 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
 }
 
 // ---------------------------------------------------------------------------
-// x-direction solve: rows staged in LDS, one thread per row
+// transpose (64 x 64 tiles through LDS).  corr = 0: Y = X^T.  corr = 1:
+// X = Y'^T where Y' is Y with the column solveFull of the x-direction solves
+// applied (serialCahnADI.c:661-678): y_j -= inv1[j] y~2 + inv2[j] y~1, j < n-2.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void xSolveKernel(const CHDesc* __restrict__ descs, int count)
+__global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict__ descs, int count,
+                                                       int corr)
 {
-    extern __shared__ double rows[];
+    __shared__ double t[kCHTr][kCHTr + 1];
     const int item = blockIdx.x;
-    const int r = findRequest(descs, count, item, 1);
+    const int r = findRequest(descs, count, item, 3);
     const CHDesc& D = descs[r];
     const int n = D.n, m = n - 2;
-    const int RB = chRowsPerBlock(n);
-    const int stride = n + 1;
-    const int row0 = (item - D.rowBase) * RB;
-    const int nrows = min(RB, n - row0);
-    double* __restrict__ X = D.X;
-
-    for (int rr = 0; rr < nrows; rr++)
-        for (int i = threadIdx.x; i < n; i += 256) rows[rr * stride + i] = X[(int64_t)(row0 + rr) * n + i];
-    __syncthreads();
-
-    if ((int)threadIdx.x < nrows) {
-        double* b = rows + threadIdx.x * stride;
-        const double* ds = D.fac;
-        const double* dl = ds + m;
-        const double* dd = dl + m;
-        const double* du = dd + m;
-        const double* dw = du + m;
-        const double* inv1 = dw + m;
-        const double* inv2 = inv1 + m;
-        // pentSolve (serialCahnADI.c:161-241)
-        double b2 = b[0] / dd[0];
-        b[0] = b2;
-        double b1 = (b[1] - dl[1] * b2) / dd[1];
-        b[1] = b1;
-        for (int i = 2; i < m; i++) {
-            const double t = b[i] - ds[i] * b2;
-            const double v = (t - dl[i] * b1) / dd[i];
-            b[i] = v;
-            b2 = b1;
-            b1 = v;
+    const int tiles = (n + kCHTr - 1) / kCHTr;
+    const int local = item - D.trBase;
+    const int R0 = (local / tiles) * kCHTr;  // source rows
+    const int C0 = (local % tiles) * kCHTr;  // source columns
+    const double* __restrict__ src = corr ? D.Y : D.X;
+    double* __restrict__ dst = corr ? D.X : D.Y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = C0 + lane;
+    double e2 = 0.0, e1 = 0.0;
+    const double* inv1 = D.fac + 5 * m;
+    const double* inv2 = inv1 + m;
+    if (corr && c < n) {
+        e2 = src[(int64_t)(n - 2) * n + c];
+        e1 = src[(int64_t)(n - 1) * n + c];
+    }
+#pragma unroll 4
+    for (int k = wave; k < kCHTr; k += 4) {
+        const int row = R0 + k;
+        if (row < n && c < n) {
+            double v = src[(int64_t)row * n + c];
+            if (corr && row < m) v = v - (inv1[row] * e2 + inv2[row] * e1);
+            t[k][lane] = v;
         }
-        // backward: b1 = b[m-1], b2 = b[m-2]
-        double a1 = b[m - 1];
-        double a0 = b[m - 2] - du[m - 2] * a1;
-        b[m - 2] = a0;
-        for (int i = m - 3; i >= 0; i--) {
-            const double t = b[i] - du[i] * a0;
-            const double v = t - dw[i] * a1;
-            b[i] = v;
-            a1 = a0;
-            a0 = v;
-        }
-        // solveEnd (serialCahnADI.c:628-655)
-        const double nx2 = b[n - 2] - (D.e * b[0] + D.a * b[n - 4] + D.b * b[n - 3]);
-        const double nx1 = b[n - 1] - (D.d * b[0] + D.e * b[1] + D.a * b[n - 3]);
-        const double x2 = D.omega[0] * nx2 + D.omega[1] * nx1;
-        const double x1 = D.omega[2] * nx2 + D.omega[3] * nx1;
-        b[n - 2] = x2;
-        b[n - 1] = x1;
-        // solveFull (serialCahnADI.c:661-678)
-        for (int i = 0; i < m; i++) b[i] = b[i] - (inv1[i] * x2 + inv2[i] * x1);
     }
     __syncthreads();
-
-    for (int rr = 0; rr < nrows; rr++)
-        for (int i = threadIdx.x; i < n; i += 256) X[(int64_t)(row0 + rr) * n + i] = rows[rr * stride + i];
+#pragma unroll 4
+    for (int k = wave; k < kCHTr; k += 4) {
+        const int row = C0 + k;  // destination row = source column
+        const int col = R0 + lane;
+        if (row < n && col < n) dst[(int64_t)row * n + col] = t[lane][k];
+    }
 }
 
 // ---------------------------------------------------------------------------
-// y-direction solve: one thread per column (coalesced across the wave)
+// column solve: one thread per column (coalesced across the wave), rows
+// streamed with a block of loads in flight ahead of the recurrence.
+// pentSolve (serialCahnADI.c:161-241) + solveEnd (:628-655); the column's
+// solveFull is applied by the next phase.  which = 0: Y, which = 1: X.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void ySolveKernel(const CHDesc* __restrict__ descs, int count)
+constexpr int PF = 16;
+
+__global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
+                                                     int which)
 {
     const int item = blockIdx.x;
     const int r = findRequest(descs, count, item, 2);
@@ -190,45 +174,75 @@ __global__ __launch_bounds__(64) void ySolveKernel(const CHDesc* __restrict__ de
     const int n = D.n, m = n - 2;
     const int col = (item - D.colBase) * kCHColBlock + (int)threadIdx.x;
     if (col >= n) return;
-    double* __restrict__ X = D.X + col;
+    double* __restrict__ Xc = (which ? D.X : D.Y) + col;
     const int64_t s = n;
-    const double* ds = D.fac;
-    const double* dl = ds + m;
-    const double* dd = dl + m;
-    const double* du = dd + m;
-    const double* dw = du + m;
+    const double* __restrict__ ds = D.fac;
+    const double* __restrict__ dl = ds + m;
+    const double* __restrict__ dd = dl + m;
+    const double* __restrict__ du = dd + m;
+    const double* __restrict__ dw = du + m;
 
-    double b2 = X[0] / dd[0];
-    X[0] = b2;
-    double b1 = (X[s] - dl[1] * b2) / dd[1];
-    X[s] = b1;
-#pragma unroll 4
-    for (int i = 2; i < m; i++) {
-        const double t = X[i * s] - ds[i] * b2;
-        const double v = (t - dl[i] * b1) / dd[i];
-        X[i * s] = v;
-        b2 = b1;
-        b1 = v;
+    // forward substitution
+    double b2 = Xc[0] / dd[0];
+    Xc[0] = b2;
+    double b1 = (Xc[s] - dl[1] * b2) / dd[1];
+    Xc[s] = b1;
+    double cur[PF], nxt[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) cur[k] = (2 + k < m) ? Xc[(2 + k) * s] : 0.0;
+    for (int i0 = 2; i0 < m; i0 += PF) {
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int i = i0 + PF + k;
+            nxt[k] = (i < m) ? Xc[i * s] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int i = i0 + k;
+            if (i < m) {
+                const double t = cur[k] - ds[i] * b2;
+                const double v = (t - dl[i] * b1) / dd[i];
+                Xc[i * s] = v;
+                b2 = b1;
+                b1 = v;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
-    double a1 = X[(m - 1) * s];
-    double a0 = X[(m - 2) * s] - du[m - 2] * a1;
-    X[(m - 2) * s] = a0;
-#pragma unroll 4
-    for (int i = m - 3; i >= 0; i--) {
-        const double t = X[i * s] - du[i] * a0;
-        const double v = t - dw[i] * a1;
-        X[i * s] = v;
-        a1 = a0;
-        a0 = v;
+    // backward substitution: b1 = x[m-1], b2 = b[m-2]
+    const double xm1 = b1;
+    double a1 = b1;
+    double a0 = b2 - du[m - 2] * a1;
+    Xc[(m - 2) * s] = a0;
+    const double xm2 = a0;
+#pragma unroll
+    for (int k = 0; k < PF; k++) cur[k] = (m - 3 - k >= 0) ? Xc[(m - 3 - k) * s] : 0.0;
+    for (int i0 = m - 3; i0 >= 0; i0 -= PF) {
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int i = i0 - PF - k;
+            nxt[k] = (i >= 0) ? Xc[i * s] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int i = i0 - k;
+            if (i >= 0) {
+                const double t = cur[k] - du[i] * a0;
+                const double v = t - dw[i] * a1;
+                Xc[i * s] = v;
+                a1 = a0;
+                a0 = v;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
-    const double x0v = a0;                // X[0]
-    const double x1v = a1;                // X[1]
-    const double xm2 = X[(n - 4) * s];
-    const double xm1 = X[(n - 3) * s];
-    const double nx2 = X[(n - 2) * s] - (D.e * x0v + D.a * xm2 + D.b * xm1);
-    const double nx1 = X[(n - 1) * s] - (D.d * x0v + D.e * x1v + D.a * xm1);
-    X[(n - 2) * s] = D.omega[0] * nx2 + D.omega[1] * nx1;
-    X[(n - 1) * s] = D.omega[2] * nx2 + D.omega[3] * nx1;
+    // solveEnd: a0 = x[0], a1 = x[1]
+    const double nx2 = Xc[(n - 2) * s] - (D.e * a0 + D.a * xm2 + D.b * xm1);
+    const double nx1 = Xc[(n - 1) * s] - (D.d * a0 + D.e * a1 + D.a * xm1);
+    Xc[(n - 2) * s] = D.omega[0] * nx2 + D.omega[1] * nx1;
+    Xc[(n - 1) * s] = D.omega[2] * nx2 + D.omega[3] * nx1;
 }
 
 // ---------------------------------------------------------------------------
@@ -278,19 +292,21 @@ hipError_t chLaunchRhs(const CHDesc* descs, int count, int totalTiles, int step,
     return hipGetLastError();
 }
 
-hipError_t chLaunchXSolve(const CHDesc* descs, int count, int totalRowBlocks, hipStream_t st)
+hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, int corr,
+                             hipStream_t st)
 {
-    if (count <= 0 || totalRowBlocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chdev::xSolveKernel, dim3(totalRowBlocks), dim3(256), kCHRowLds, st, descs,
-                       count);
+    if (count <= 0 || totalTiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(chdev::transposeKernel, dim3(totalTiles), dim3(256), 0, st, descs, count,
+                       corr);
     return hipGetLastError();
 }
 
-hipError_t chLaunchYSolve(const CHDesc* descs, int count, int totalColBlocks, hipStream_t st)
+hipError_t chLaunchColSolve(const CHDesc* descs, int count, int totalColBlocks, int which,
+                            hipStream_t st)
 {
     if (count <= 0 || totalColBlocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chdev::ySolveKernel, dim3(totalColBlocks), dim3(kCHColBlock), 0, st, descs,
-                       count);
+    hipLaunchKernelGGL(chdev::colSolveKernel, dim3(totalColBlocks), dim3(kCHColBlock), 0, st,
+                       descs, count, which);
     return hipGetLastError();
 }
 

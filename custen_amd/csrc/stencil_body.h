@@ -33,7 +33,7 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
     const int x0 = tx * kTW;
     const bool inside = x0 + kTW <= nx;
     const bool vecIn = a.vecIn && inside;
-    const bool fullRow = FAST || (a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1);
+    const bool fullRow = a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1;
 
     const int y00 = tyg * a.chunks * TH;
     if (y00 >= a.ny) return;
@@ -87,15 +87,14 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
     }
 }
 
-// True when column strip tx and its column halo lie inside [0, nx), its rows
-// are pair-aligned and fully written, and one wrap suffices for row indices:
-// such strips run the FAST instantiation.
+// True when column strip tx is a full 128 columns of pair-aligned rows and
+// one wrap suffices for row indices: such strips run the FAST instantiation
+// (only a ragged last strip, or unaligned data, takes the general path).
 __host__ __device__ inline bool stripIsInterior(const StencilLaunch& a, int tx)
 {
     const int x0 = tx * kTW;
-    const int R = a.H - 1 - a.L;
-    return a.vecIn && a.vecOut && x0 - a.L >= 0 && x0 + kTW + R <= a.nx && x0 >= a.wx0 &&
-           x0 + kTW <= a.wx1 && (a.rows.wrap ? a.rows.rowWrap >= a.V : true);
+    return a.vecIn && a.vecOut && x0 + kTW <= a.nx &&
+           (a.rows.wrap ? a.rows.rowWrap >= a.V : true);
 }
 
 // Strip g of a single-grid launch -> (tx, tyg).

@@ -149,9 +149,9 @@ __device__ __forceinline__ int mapRowNear(const RowMap& m, int gy)
     return gy < m.rowMin ? m.rowMin : (gy > m.rowMax ? m.rowMax : gy);
 }
 
-// FAST: the strip and its column halo lie inside [0, nx) with pair-aligned
-// rows, and a row index needs at most one wrap (ny >= window height): no
-// column mapping and no division in the address arithmetic.
+// FAST: the strip is a full 128 columns of pair-aligned rows, and row and
+// halo-column indices need at most one wrap (ny >= window height, nx >= 128
+// > any extent): no division in the address arithmetic.
 template <typename T, int NR, bool FAST>
 __device__ __forceinline__ void loadRows(StagedRows<T, NR>& s, const T* __restrict__ in,
                                          int64_t ldIn, const RowMap& rows, int y0, int T_, int x0,
@@ -173,8 +173,14 @@ __device__ __forceinline__ void loadRows(StagedRows<T, NR>& s, const T* __restri
                 s.body[i].y = grow[mapCol(gx + 1, nx, xWrap)];
             }
             if (lane < L + R) {
-                const int hx = lane < L ? x0 - L + lane : x0 + kTW + (lane - L);
-                s.halo[i] = FAST ? grow[hx] : grow[mapCol(hx, nx, xWrap)];
+                int hx = lane < L ? x0 - L + lane : x0 + kTW + (lane - L);
+                if (FAST) {
+                    if (xWrap) hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
+                    else hx = hx < 0 ? 0 : (hx >= nx ? nx - 1 : hx);
+                } else {
+                    hx = mapCol(hx, nx, xWrap);
+                }
+                s.halo[i] = grow[hx];
             }
         }
     }

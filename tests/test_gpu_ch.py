@@ -80,3 +80,21 @@ def test_server_admission_budget(gpu):
         srv.submit(api.ch_params(256), 3, ic(256, 9))
     assert ei.value.code == 5
     srv.close()
+
+
+def test_modes_interleave(gpu):
+    """Fused (mode 0) and reference-shaped (mode 1) steps can alternate on one
+    simulation; the result equals the oracle after the total step count."""
+    n = 96
+    c0 = ic(n, 3)
+    s = api.CHSolver(api.ch_params(n), c0)
+    s.run(3, mode=0)
+    s.run(2, mode=1)
+    s.run(2, mode=0)
+    c, co = s.field()
+    s.close()
+    ref = oracle.CH(n)
+    rc, rco = c0.ravel().copy(), c0.ravel().copy()
+    ref.step(rc, rco, steps=7)
+    assert mismatches(c, rc.reshape(n, n)) == 0
+    assert mismatches(co, rco.reshape(n, n)) == 0
