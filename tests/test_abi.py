@@ -62,3 +62,30 @@ def test_uncreated_handle_is_rejected():
 def test_builtin_names():
     with pytest.raises(api.CustenError):
         api.custenGetBuiltinFunction("no_such_function", 0)
+
+
+def test_ch_validation_without_gpu():
+    L = api.lib()
+    p = api.ch_params(4)  # below the minimum grid
+    import numpy as np
+    c0 = np.zeros(16)
+    assert not L.custenCHCreate(0, ctypes.byref(p), c0.ctypes.data)
+    assert L.custenGetLastError() == 1
+    api.clear_error()
+    assert L.custenCHFootprint(64) == 4 * 64 * 64 * 8 + 7 * 62 * 8
+    assert not L.custenCHServerCreate(0, 0, 4)
+    assert L.custenGetLastError() == 1
+    api.clear_error()
+
+
+def test_batch_validation_without_gpu():
+    L = api.lib()
+    nx = (ctypes.c_int * 1)(64)
+    ptr = (ctypes.c_void_p * 1)(0)
+    assert not L.custenBatchCreate(0, 1, nx, nx, ptr, ptr, None, 5, 2, 5, 2, 0)
+    assert L.custenGetLastError() == 1
+    api.clear_error()
+    w = (ctypes.c_double * 49)()
+    assert not L.custenBatchCreate(0, 1, nx, nx, ptr, ptr, w, 7, 3, 7, 3, 0)
+    assert L.custenGetLastError() == 4  # shape not supported for batches
+    api.clear_error()
