@@ -146,6 +146,7 @@ _SIGS.update({
     "custenDestroy2DF32": (None, [_H]),
     "custenSetStream": (None, [_H, c_void_p]),
     "custenSetTileConfig": (None, [_H, c_int, c_int]),
+    "custenSetTileConfigEx": (None, [_H, c_int, c_int, c_int]),
     "custenBatchCreate": (c_void_p, [c_int, c_int, _P, _P, _P, _P, _P, c_int, c_int, c_int,
                                      c_int, c_int]),  # noqa: E501
     "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
@@ -317,9 +318,9 @@ def custenSetForceIndirect(h: cuSten_t, force: bool) -> None:
     check("custenSetForceIndirect")
 
 
-def custenSetTileConfig(h: cuSten_t, rowsPerWave: int, chunks: int) -> None:
-    lib().custenSetTileConfig(_handle(h), rowsPerWave, chunks)
-    check("custenSetTileConfig")
+def custenSetTileConfig(h: cuSten_t, rowsPerWave: int, chunks: int, colsPerLane: int = 0) -> None:
+    lib().custenSetTileConfigEx(_handle(h), rowsPerWave, chunks, colsPerLane)
+    check("custenSetTileConfigEx")
 
 
 def custenSetStream(h: cuSten_t, stream: int) -> None:

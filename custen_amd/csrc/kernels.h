@@ -52,6 +52,7 @@ struct StencilLaunch {
     Xform xform;
     int rowsPerWave;         // 0 = default
     int chunks;              // row chunks per workgroup, 0 = default
+    int colsPerLane;         // 1 or 2 (0 = default 2)
     int vecIn, vecOut;       // set by launchStencil: rows are pair-aligned
     // column strips handled by one launch: interior launches cover
     // tx = stripTx0 .. stripTx0 + nStripTx - 1, edge launches the listed tx
@@ -75,7 +76,7 @@ struct FunLaunch {
     int xWrap;
     RowMap rows;
     int wx0, wx1, wy0, wy1, yGlobal0;
-    int rowsPerWave, chunks;
+    int rowsPerWave, chunks, colsPerLane;
     int vecIn;
 };
 

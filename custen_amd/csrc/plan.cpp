@@ -412,6 +412,7 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
         s.xform = Xform::Identity;
         s.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
         s.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
+        s.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
         return launchStencil(s, st);
     }
     FunLaunch f{};
@@ -439,6 +440,7 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
     f.yGlobal0 = r.yGlobal0;
     f.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
     f.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
+    f.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
     return launchFun(f, st);
 }
 
@@ -634,7 +636,8 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
         r.yGlobal0 = 0;
         // First resident Compute of a large grid: pick the tile configuration
         // (bit-identical results for every candidate; synchronous once).
-        if (!p->tuned && p->rowsPerWave == 0 && p->chunks == 0 && tunerEnabled() &&
+        if (!p->tuned && p->rowsPerWave == 0 && p->chunks == 0 && p->colsPerLane == 0 &&
+            tunerEnabled() &&
             (int64_t)h->nx * h->ny >= (int64_t(1) << 20) && !streamCapturing(st)) {
             TuneKey k;
             k.kind = 0;
@@ -647,12 +650,14 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             k.ny = h->ny;
             TuneVal v;
             if (!tunerLookup(k, v)) {
-                static const std::vector<TuneVal> cands = {{4, 1}, {4, 2}, {4, 4}, {8, 1},
-                                                           {8, 2}, {8, 4}, {8, 8}, {8, 16}};
+                static const std::vector<TuneVal> cands = {
+                    {4, 1, 2}, {4, 2, 2}, {4, 4, 2}, {8, 1, 2},  {8, 2, 2},  {8, 4, 2},
+                    {8, 8, 2}, {8, 2, 1}, {8, 4, 1}, {16, 1, 1}, {16, 2, 1}, {16, 4, 1}};
                 auto run = [&](const TuneVal& c) {
                     Plan tmp = *p;
                     tmp.rowsPerWave = c.rpw;
                     tmp.chunks = c.chunks;
+                    tmp.colsPerLane = c.cpl;
                     return launchRegion(h, &tmp, r, params, st);
                 };
                 e = tunerMeasure(st, run, cands, 3, v);
@@ -665,6 +670,7 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             p->tuned = true;
             p->tunedRpw = v.rpw;
             p->tunedChunks = v.chunks;
+            p->tunedCpl = v.cpl;
         }
         e = launchRegion(h, p, r, params, st);
         if (e != hipSuccess) setHipError(e, fn);
@@ -986,22 +992,38 @@ void custenSetStream(cuSten_t* h, hipStream_t stream)
     p->hasUserStream = true;
 }
 
-void custenSetTileConfig(cuSten_t* h, int rowsPerWave, int chunksPerGroup)
+void custenSetTileConfigEx(cuSten_t* h, int rowsPerWave, int chunksPerGroup, int columnsPerLane)
 {
-    Plan* p = planOf(h, "custenSetTileConfig");
+    Plan* p = planOf(h, "custenSetTileConfigEx");
     if (!p) return;
-    if (rowsPerWave != 0 && rowsPerWave != 4 && rowsPerWave != 8) {
-        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetTileConfig: rowsPerWave must be 0, 4 or 8");
+    if (columnsPerLane != 0 && columnsPerLane != 1 && columnsPerLane != 2) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetTileConfigEx: columnsPerLane must be 0, 1 or 2");
+        return;
+    }
+    const bool ok = rowsPerWave == 0 ||
+                    (columnsPerLane == 1 ? (rowsPerWave == 8 || rowsPerWave == 16)
+                                         : (rowsPerWave == 4 || rowsPerWave == 8));
+    if (!ok) {
+        setError(CUSTEN_ERROR_INVALID_VALUE,
+                 "custenSetTileConfigEx: rowsPerWave must be 4 or 8 (2 columns per lane), "
+                 "8 or 16 (1 column per lane), or 0");
         return;
     }
     if (chunksPerGroup < 0) {
-        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetTileConfig: chunksPerGroup must be >= 0");
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetTileConfigEx: chunksPerGroup must be >= 0");
         return;
     }
     p->rowsPerWave = rowsPerWave;
     p->chunks = chunksPerGroup;
-    if (rowsPerWave == 0 && chunksPerGroup == 0) p->tuned = false;  // re-tune on next Compute
+    p->colsPerLane = columnsPerLane;
+    if (rowsPerWave == 0 && chunksPerGroup == 0 && columnsPerLane == 0)
+        p->tuned = false;  // re-tune on next Compute
     h->mem_shared = stencilLdsBytes(p->dtype, p->H, p->V, rowsPerWave);
+}
+
+void custenSetTileConfig(cuSten_t* h, int rowsPerWave, int chunksPerGroup)
+{
+    custenSetTileConfigEx(h, rowsPerWave, chunksPerGroup, 0);
 }
 
 void custenSetForceIndirect(cuSten_t* h, int force)

@@ -30,10 +30,11 @@ struct Plan {
     bool zeroRight = false;     // x-np: right R columns written as 0.0
     hipStream_t userStream = nullptr;
     bool hasUserStream = false;
-    int rowsPerWave = 0;        // set by custenSetTileConfig (0: autotuned)
+    int rowsPerWave = 0;        // set by custenSetTileConfig[Ex] (0: autotuned)
     int chunks = 0;
+    int colsPerLane = 0;
     bool tuned = false;         // autotuner result for resident launches
-    int tunedRpw = 0, tunedChunks = 0;
+    int tunedRpw = 0, tunedChunks = 0, tunedCpl = 0;
     // device copy of host-resident weights / coefficients
     double* paramDev = nullptr;
     size_t paramCap = 0;

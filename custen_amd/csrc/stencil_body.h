@@ -10,16 +10,18 @@
 namespace custen {
 namespace dev {
 
-// Process strip (tx, tyg) of the region described by `a`: column strip tx,
-// row chunks [tyg * chunks, (tyg + 1) * chunks).  `w` holds the H*V weights, `tile` the LDS staging buffer of
-// (4*RPW + V - 1) x ldsRowLength(H) elements.
-template <typename T, int H, int V, int RPW, Xform X, bool FAST>
+// Process strip (tx, tyg) of the region described by `a`: column strip tx
+// (64 * CPL columns), row chunks [tyg * chunks, (tyg + 1) * chunks).  `w` holds
+// the H*V weights, `tile` the LDS staging buffer of
+// (4*RPW + V - 1) x ldsRowLengthW(64 * CPL, H) elements.
+template <typename T, int H, int V, int RPW, Xform X, bool FAST, int CPL = 2>
 __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T (&w)[H * V],
                                                  T* __restrict__ tile, int tx, int tyg)
 {
     constexpr int TH = kWaves * RPW;
     constexpr int NR = TH + V - 1;
-    constexpr int LD = ldsRowLength(H);
+    constexpr int TW = kLanes * CPL;
+    constexpr int LD = ldsRowLengthW(TW, H);
 
     const T* __restrict__ in = static_cast<const T*>(a.in);
     T* __restrict__ out = static_cast<T*>(a.out);
@@ -30,46 +32,49 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
     const int R = H - 1 - L;
     const int LP = (L + 1) & ~1;
     const int nx = a.nx;
-    const int x0 = tx * kTW;
-    const bool inside = x0 + kTW <= nx;
-    const bool vecIn = a.vecIn && inside;
-    const bool fullRow = a.vecOut && inside && x0 >= a.wx0 && x0 + kTW <= a.wx1;
+    const int x0 = tx * TW;
+    const bool inside = x0 + TW <= nx;
+    const bool vecIn = (CPL == 1 || a.vecIn) && inside;
+    const bool fullRow = (CPL == 1 || a.vecOut) && inside && x0 >= a.wx0 && x0 + TW <= a.wx1;
 
     const int y00 = tyg * a.chunks * TH;
     if (y00 >= a.ny) return;
-    StagedRows<T, NR> st;
-    loadRows<T, NR, FAST>(st, in, a.ldIn, a.rows, y00, a.T, x0, nx, L, R, a.xWrap, vecIn, wave, lane);
+    StagedRows<T, NR, CPL> st;
+    loadRows<T, NR, FAST, CPL>(st, in, a.ldIn, a.rows, y00, a.T, x0, nx, L, R, a.xWrap, vecIn,
+                               wave, lane);
 
     for (int c = 0; c < a.chunks; c++) {
         const int y0 = y00 + c * TH;
         if (c > 0) __syncthreads();  // the previous chunk's compute is done with LDS
-        storeRows<T, X, NR>(st, tile, LD, L, R, LP, wave, lane);
+        storeRows<T, X, NR, CPL>(st, tile, LD, L, R, LP, wave, lane);
         __syncthreads();
         const bool more = (c + 1 < a.chunks) && (y0 + TH < a.ny);
         // prefetch the next chunk's rows; they stay in flight during the compute
         if (more)
-            loadRows<T, NR, FAST>(st, in, a.ldIn, a.rows, y0 + TH, a.T, x0, nx, L, R, a.xWrap, vecIn,
-                            wave, lane);
+            loadRows<T, NR, FAST, CPL>(st, in, a.ldIn, a.rows, y0 + TH, a.T, x0, nx, L, R, a.xWrap,
+                                       vecIn, wave, lane);
 
-        T acc[RPW][2];
+        T acc[RPW][CPL];
 #pragma unroll
-        for (int r = 0; r < RPW; r++) acc[r][0] = acc[r][1] = T(0);
+        for (int r = 0; r < RPW; r++)
+#pragma unroll
+            for (int e = 0; e < CPL; e++) acc[r][e] = T(0);
 
-        const T* base = tile + (wave * RPW) * LD + (LP - L) + 2 * lane;
+        const T* base = tile + (wave * RPW) * LD + (LP - L) + CPL * lane;
 #pragma unroll
         for (int ir = 0; ir < RPW + V - 1; ir++) {
-            T v[H + 1];
+            T v[H + CPL - 1];
 #pragma unroll
-            for (int k = 0; k <= H; k++) v[k] = base[ir * LD + k];
+            for (int k = 0; k < H + CPL - 1; k++) v[k] = base[ir * LD + k];
 #pragma unroll
             for (int r = 0; r < RPW; r++) {
                 const int j = ir - r;
                 if (j >= 0 && j < V) {
 #pragma unroll
-                    for (int i = 0; i < H; i++) {
-                        acc[r][0] = fmaT(w[j * H + i], v[i], acc[r][0]);
-                        acc[r][1] = fmaT(w[j * H + i], v[i + 1], acc[r][1]);
-                    }
+                    for (int i = 0; i < H; i++)
+#pragma unroll
+                        for (int e = 0; e < CPL; e++)
+                            acc[r][e] = fmaT(w[j * H + i], v[i + e], acc[r][e]);
                 }
             }
         }
@@ -80,8 +85,8 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
             if (gy >= a.ny) break;
             const int gyg = a.yGlobal0 + gy;
             if (gyg < a.wy0 || gyg >= a.wy1) continue;
-            storePair<T>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r][0], acc[r][1], fullRow,
-                         nx, a.wx0, a.wx1, a.zeroRight, R);
+            storeLane<T, CPL>(out + (int64_t)gy * a.ldOut, x0 + CPL * lane, acc[r], fullRow, nx,
+                              a.wx0, a.wx1, a.zeroRight, R);
         }
         if (!more) break;
     }
@@ -90,10 +95,11 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
 // True when column strip tx is a full 128 columns of pair-aligned rows and
 // one wrap suffices for row indices: such strips run the FAST instantiation
 // (only a ragged last strip, or unaligned data, takes the general path).
-__host__ __device__ inline bool stripIsInterior(const StencilLaunch& a, int tx)
+__host__ __device__ inline bool stripIsInterior(const StencilLaunch& a, int tx, int cpl = 2)
 {
-    const int x0 = tx * kTW;
-    return a.vecIn && a.vecOut && x0 + kTW <= a.nx &&
+    const int tw = kLanes * cpl;
+    const int x0 = tx * tw;
+    return (cpl == 1 || (a.vecIn && a.vecOut)) && x0 + tw <= a.nx &&
            (a.rows.wrap ? a.rows.rowWrap >= a.V : true);
 }
 

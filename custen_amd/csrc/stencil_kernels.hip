@@ -32,24 +32,25 @@ namespace dev {
 #ifndef CUSTEN_OCC_F64_LARGE
 #define CUSTEN_OCC_F64_LARGE 3
 #endif
-template <typename T, int H, int V, int RPW>
+template <typename T, int H, int V, int RPW, int CPL>
 struct FixedOccupancy {
     static constexpr int value =
-        (sizeof(T) == 8 && H * V >= 25 && RPW == 4) ? CUSTEN_OCC_F64_LARGE : 1;
+        (sizeof(T) == 8 && H * V >= 25 && RPW == 4 && CPL == 2) ? CUSTEN_OCC_F64_LARGE : 1;
 };
 
-template <typename T, int H, int V, int RPW, Xform X, bool FAST>
+template <typename T, int H, int V, int RPW, Xform X, bool FAST, int CPL>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(FixedOccupancy<T, H, V, RPW>::value, 8))) void stencilFixed(
+__attribute__((amdgpu_waves_per_eu(FixedOccupancy<T, H, V, RPW, CPL>::value, 8))) void stencilFixed(
     StencilLaunch a)
 {
-    __shared__ __attribute__((aligned(16))) T tile[(kWaves * RPW + V - 1) * ldsRowLength(H)];
+    __shared__ __attribute__((aligned(16)))
+    T tile[(kWaves * RPW + V - 1) * ldsRowLengthW(kLanes * CPL, H)];
     T w[H * V];
 #pragma unroll
     for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
     int tx, tyg;
     stripOf(a, FAST, xcdRemap(blockIdx.x, gridDim.x), tx, tyg);
-    stencilStripImpl<T, H, V, RPW, X, FAST>(a, w, tile, tx, tyg);
+    stencilStripImpl<T, H, V, RPW, X, FAST, CPL>(a, w, tile, tx, tyg);
 }
 
 // ---------------------------------------------------------------------------
@@ -322,18 +323,19 @@ int gridSize(int nx, int ny, int TH, int chunks)
 
 // Interior column strips run the FAST kernel, the (at most four) edge strips
 // the general one.
-template <typename T, int H, int V, int RPW, Xform X>
+template <typename T, int H, int V, int RPW, Xform X, int CPL = 2>
 hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
 {
     StencilLaunch s = in;
-    const int tilesX = (s.nx + dev::kTW - 1) / dev::kTW;
+    const int tw = dev::kLanes * CPL;
+    const int tilesX = (s.nx + tw - 1) / tw;
     const int rowsPerGroup = dev::kWaves * RPW * s.chunks;
     const int tilesY = (s.ny + rowsPerGroup - 1) / rowsPerGroup;
     int lo = 0, hi = tilesX;  // interior range [lo, hi)
-    while (lo < hi && !dev::stripIsInterior(s, lo)) lo++;
-    while (hi > lo && !dev::stripIsInterior(s, hi - 1)) hi--;
+    while (lo < hi && !dev::stripIsInterior(s, lo, CPL)) lo++;
+    while (hi > lo && !dev::stripIsInterior(s, hi - 1, CPL)) hi--;
     for (int t = lo; t < hi; t++)
-        if (!dev::stripIsInterior(s, t)) return hipErrorInvalidValue;  // not contiguous
+        if (!dev::stripIsInterior(s, t, CPL)) return hipErrorInvalidValue;  // not contiguous
     StencilLaunch e = s;
     e.nStripTx = 0;
     for (int t = 0; t < tilesX; t++) {
@@ -346,7 +348,7 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     if (e.nStripTx > 0) {
         e.chunks = 1;
         const int tilesYe = (s.ny + dev::kWaves * RPW - 1) / (dev::kWaves * RPW);
-        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, false>),
+        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, false, CPL>),
                            dim3(e.nStripTx * tilesYe), dim3(dev::kThreads), 0, st, e);
         hipError_t err = hipGetLastError();
         if (err != hipSuccess) return err;
@@ -354,7 +356,7 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     if (hi > lo) {
         s.stripTx0 = lo;
         s.nStripTx = hi - lo;
-        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, true>),
+        hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, true, CPL>),
                            dim3(s.nStripTx * tilesY), dim3(dev::kThreads), 0, st, s);
         return hipGetLastError();
     }
@@ -387,6 +389,13 @@ template <typename T, Xform X>
 hipError_t dispatchTyped(const StencilLaunch& s, hipStream_t st)
 {
     const int rpw = s.rowsPerWave;
+    // one column per lane (deeper row blocking) for the xy shapes
+    if (s.colsPerLane == 1) {
+        if (s.H == 5 && s.V == 5)
+            return rpw == 16 ? runFixed<T, 5, 5, 16, X, 1>(s, st) : runFixed<T, 5, 5, 8, X, 1>(s, st);
+        if (s.H == 3 && s.V == 3)
+            return rpw == 16 ? runFixed<T, 3, 3, 16, X, 1>(s, st) : runFixed<T, 3, 3, 8, X, 1>(s, st);
+    }
 #define CUSTEN_CASE(HH, VV)                                                      \
     if (s.H == HH && s.V == VV) {                                                \
         if (rpw == 4) return runFixed<T, HH, VV, 4, X>(s, st);                   \
@@ -411,8 +420,12 @@ hipError_t launchStencil(const StencilLaunch& in, hipStream_t st)
     StencilLaunch s = in;
     if (s.nx <= 0 || s.ny <= 0) return hipSuccess;
     if (s.H < 1 || s.V < 1 || s.H > 65 || s.V > 65) return hipErrorInvalidValue;
-    if (s.rowsPerWave != 4 && s.rowsPerWave != 8)
-        s.rowsPerWave = (s.dtype == Dtype::F64 && s.H * s.V >= 25) ? 4 : kDefaultRPW;
+    if (s.colsPerLane != 1) s.colsPerLane = 2;
+    if (s.colsPerLane == 1) {
+        if (s.rowsPerWave != 8 && s.rowsPerWave != 16) s.rowsPerWave = 16;
+    } else if (s.rowsPerWave != 4 && s.rowsPerWave != 8) {
+        s.rowsPerWave = kDefaultRPW;
+    }
     if (s.chunks <= 0) s.chunks = kDefaultChunks;
     if (s.dtype == Dtype::F64) {
         s.vecIn = pairAligned<double>(s.in, s.ldIn);
@@ -463,6 +476,7 @@ hipError_t launchFun(const FunLaunch& in, hipStream_t st)
         s.xform = f.builtin == Builtin::XYCubeMinus ? Xform::CubeMinus : Xform::Identity;
         s.rowsPerWave = f.rowsPerWave;
         s.chunks = f.chunks;
+        s.colsPerLane = f.colsPerLane;
         return launchStencil(s, st);
     }
 

@@ -83,8 +83,28 @@ __device__ __forceinline__ int xcdRemap(int g, int G)
 
 // LDS row geometry for a stencil of width H with left extent L:
 // LDS column k holds global column x0 - LP + k, where LP = L rounded up to
-// even so that the 128-column body starts 16-byte aligned.
-__host__ __device__ constexpr int ldsRowLength(int H) { return (kTW + H + 1) & ~1; }
+// even so that the strip body starts 16-byte aligned.  A strip is 64 lanes x
+// CPL columns wide (CPL = columns per lane, 1 or 2).
+__host__ __device__ constexpr int ldsRowLengthW(int TW, int H) { return (TW + H + 1) & ~1; }
+__host__ __device__ constexpr int ldsRowLength(int H) { return ldsRowLengthW(kTW, H); }
+
+// The per-lane element group of a strip row: one value (CPL = 1) or an
+// aligned pair (CPL = 2).
+template <typename T, int CPL> struct LaneVec;
+template <typename T> struct LaneVec<T, 2> {
+    using type = typename Vec2<T>::type;
+    __device__ static T get(const type& v, int e) { return e == 0 ? v.x : v.y; }
+    __device__ static void set(type& v, int e, T x)
+    {
+        if (e == 0) v.x = x;
+        else v.y = x;
+    }
+};
+template <typename T> struct LaneVec<T, 1> {
+    using type = T;
+    __device__ static T get(const type& v, int) { return v; }
+    __device__ static void set(type& v, int, T x) { v = x; }
+};
 
 // Stage one input row (all of [x0-L, x0+kTW+R)) into LDS.  One wave per row.
 template <typename T, Xform X>
@@ -130,10 +150,10 @@ __device__ __forceinline__ void stageTile(T* __restrict__ tile, int LD, int nrow
 // Rows of one chunk held in registers between their global load and their
 // LDS store, so that a wave issues all its row loads before it waits on any
 // of them (and, with prefetch, while the previous chunk is being computed).
-template <typename T, int NR>
+template <typename T, int NR, int CPL = 2>
 struct StagedRows {
     static constexpr int PER = (NR + kWaves - 1) / kWaves;  // rows per wave
-    typename Vec2<T>::type body[PER];
+    typename LaneVec<T, CPL>::type body[PER];
     T halo[PER];
 };
 
@@ -152,28 +172,30 @@ __device__ __forceinline__ int mapRowNear(const RowMap& m, int gy)
 // FAST: the strip is a full 128 columns of pair-aligned rows, and row and
 // halo-column indices need at most one wrap (ny >= window height, nx >= 128
 // > any extent): no division in the address arithmetic.
-template <typename T, int NR, bool FAST>
-__device__ __forceinline__ void loadRows(StagedRows<T, NR>& s, const T* __restrict__ in,
+template <typename T, int NR, bool FAST, int CPL = 2>
+__device__ __forceinline__ void loadRows(StagedRows<T, NR, CPL>& s, const T* __restrict__ in,
                                          int64_t ldIn, const RowMap& rows, int y0, int T_, int x0,
                                          int nx, int L, int R, int xWrap, bool vecIn, int wave,
                                          int lane)
 {
-    using V2 = typename Vec2<T>::type;
+    using LV = LaneVec<T, CPL>;
+    using VT = typename LV::type;
+    constexpr int TW = kLanes * CPL;
 #pragma unroll
-    for (int i = 0; i < StagedRows<T, NR>::PER; i++) {
+    for (int i = 0; i < StagedRows<T, NR, CPL>::PER; i++) {
         const int r = wave + i * kWaves;
         if (r < NR) {
             const int sy = FAST ? mapRowNear(rows, y0 - T_ + r) : mapRow(rows, y0 - T_ + r);
             const T* grow = in + (int64_t)sy * ldIn;
-            const int gx = x0 + 2 * lane;
+            const int gx = x0 + CPL * lane;
             if (FAST || vecIn) {
-                s.body[i] = *reinterpret_cast<const V2*>(grow + gx);
+                s.body[i] = *reinterpret_cast<const VT*>(grow + gx);
             } else {
-                s.body[i].x = grow[mapCol(gx, nx, xWrap)];
-                s.body[i].y = grow[mapCol(gx + 1, nx, xWrap)];
+#pragma unroll
+                for (int e = 0; e < CPL; e++) LV::set(s.body[i], e, grow[mapCol(gx + e, nx, xWrap)]);
             }
             if (lane < L + R) {
-                int hx = lane < L ? x0 - L + lane : x0 + kTW + (lane - L);
+                int hx = lane < L ? x0 - L + lane : x0 + TW + (lane - L);
                 if (FAST) {
                     if (xWrap) hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
                     else hx = hx < 0 ? 0 : (hx >= nx ? nx - 1 : hx);
@@ -186,25 +208,50 @@ __device__ __forceinline__ void loadRows(StagedRows<T, NR>& s, const T* __restri
     }
 }
 
-template <typename T, Xform X, int NR>
-__device__ __forceinline__ void storeRows(const StagedRows<T, NR>& s, T* __restrict__ tile, int LD,
-                                          int L, int R, int LP, int wave, int lane)
+template <typename T, Xform X, int NR, int CPL = 2>
+__device__ __forceinline__ void storeRows(const StagedRows<T, NR, CPL>& s, T* __restrict__ tile,
+                                          int LD, int L, int R, int LP, int wave, int lane)
 {
-    using V2 = typename Vec2<T>::type;
+    using LV = LaneVec<T, CPL>;
+    using VT = typename LV::type;
+    constexpr int TW = kLanes * CPL;
 #pragma unroll
-    for (int i = 0; i < StagedRows<T, NR>::PER; i++) {
+    for (int i = 0; i < StagedRows<T, NR, CPL>::PER; i++) {
         const int r = wave + i * kWaves;
         if (r < NR) {
             T* lrow = tile + r * LD;
-            V2 o;
-            o.x = xformValue<T, X>(s.body[i].x);
-            o.y = xformValue<T, X>(s.body[i].y);
-            *reinterpret_cast<V2*>(lrow + LP + 2 * lane) = o;
+            VT o;
+#pragma unroll
+            for (int e = 0; e < CPL; e++) LV::set(o, e, xformValue<T, X>(LV::get(s.body[i], e)));
+            *reinterpret_cast<VT*>(lrow + LP + CPL * lane) = o;
             if (lane < L + R) {
-                const int hx = lane < L ? LP - L + lane : LP + kTW + (lane - L);
+                const int hx = lane < L ? LP - L + lane : LP + TW + (lane - L);
                 lrow[hx] = xformValue<T, X>(s.halo[i]);
             }
         }
+    }
+}
+
+// Write one output row's CPL values for this lane, honouring the write window.
+template <typename T, int CPL>
+__device__ __forceinline__ void storeLane(T* __restrict__ orow, int gx, const T* v, bool fullRow,
+                                          int nx, int wx0, int wx1, int zeroRight, int R)
+{
+    using LV = LaneVec<T, CPL>;
+    using VT = typename LV::type;
+    if (fullRow) {
+        VT o;
+#pragma unroll
+        for (int e = 0; e < CPL; e++) LV::set(o, e, v[e]);
+        __builtin_nontemporal_store(o, reinterpret_cast<VT*>(orow + gx));
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+        const int x = gx + e;
+        if (x >= nx) continue;
+        if (x >= wx0 && x < wx1) orow[x] = v[e];
+        else if (zeroRight && x >= nx - R) orow[x] = T(0);
     }
 }
 

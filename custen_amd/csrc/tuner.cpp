@@ -39,8 +39,8 @@ void loadLocked()
     if (!f) return;
     TuneKey k;
     TuneVal v;
-    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d", &k.kind, &k.dtype, &k.H, &k.V, &k.xform,
-                       &k.nx, &k.ny, &k.count, &v.rpw, &v.chunks) == 10)
+    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d", &k.kind, &k.dtype, &k.H, &k.V,
+                       &k.xform, &k.nx, &k.ny, &k.count, &v.rpw, &v.chunks, &v.cpl) == 11)
         g_table[k] = v;
     std::fclose(f);
 }
@@ -78,8 +78,8 @@ void tunerStore(const TuneKey& k, const TuneVal& v)
     if (!path) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", k.kind, k.dtype, k.H, k.V, k.xform, k.nx,
-                 k.ny, k.count, v.rpw, v.chunks);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d\n", k.kind, k.dtype, k.H, k.V, k.xform, k.nx,
+                 k.ny, k.count, v.rpw, v.chunks, v.cpl);
     std::fclose(f);
 }
 

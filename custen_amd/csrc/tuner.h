@@ -21,6 +21,7 @@ struct TuneKey {
 struct TuneVal {
     int rpw = 8;
     int chunks = 4;
+    int cpl = 2;  // columns per lane
 };
 
 bool tunerEnabled();

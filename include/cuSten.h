@@ -278,9 +278,13 @@ void custenDestroy2DF32(cuSten_t* pt_cuSten);
  * handle's first stream).  Passing 0 selects the null stream. */
 void custenSetStream(cuSten_t* pt_cuSten, hipStream_t stream);
 
-/* Tile configuration of the stencil kernel for a handle: rows per wave
- * (1..16) and row-chunks per workgroup (>= 1).  0 keeps the autotuned value. */
+/* Tile configuration of the stencil kernel for a handle (results do not
+ * depend on it).  columnsPerLane 2 (default): rowsPerWave 4 or 8; columnsPerLane
+ * 1: rowsPerWave 8 or 16.  chunksPerGroup >= 1 row chunks per workgroup.  All
+ * zero: autotune on the next Compute. */
 void custenSetTileConfig(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGroup);
+void custenSetTileConfigEx(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGroup,
+                           int columnsPerLane);
 
 /* Fill out[0..n) with the SplitMix64 uniform stream (value i from state
  * seed + (i+1)*0x9E3779B97F4A7C15, u = (z >> 11) 2^-53, a + (b - a) u). Host. */
