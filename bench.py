@@ -145,6 +145,9 @@ def bench_stencil_8192(args, torch, api, extra):
     api.cuStenCompute2DXYp(h, api.DEVICE)  # autotunes the tile configuration once
     torch.cuda.synchronize()
     ms = timer.run(compute_fn(api, "xy_p_bih", h), args.steps, args.warmup)
+    rpw, chunks, cpl = api.custenGetTileConfig(h)
+    extra["headline_tile_config"] = {"rows_per_wave": rpw, "chunks": chunks,
+                                     "columns_per_lane": cpl}
     destroy(api, "xy_p_bih", h)
     # parity spot check of the measured output against the oracle (rows 0..255)
     ref = oracle.apply(host[:260].copy(), wbh, 5, 2, 5, 2, False)
@@ -305,7 +308,7 @@ def bench_ch(args, torch, api, extra):
                     "one request at a time",
            "fused_one_at_a_time_solves_per_s": round(count / t_single, 2)}
     budget = int(args.ch_budget_gb * (1 << 30))
-    for load in (0.5, 2.0):
+    for load in (1.0, 4.0, 16.0):
         arrivals = workloads.ch_requests(count, rate_per_s=load * plain_rate)
         srv = api.CHServer(budget_bytes=budget, max_slots=args.ch_slots)
         pending = list(enumerate(arrivals))

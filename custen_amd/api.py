@@ -147,6 +147,8 @@ _SIGS.update({
     "custenSetStream": (None, [_H, c_void_p]),
     "custenSetTileConfig": (None, [_H, c_int, c_int]),
     "custenSetTileConfigEx": (None, [_H, c_int, c_int, c_int]),
+    "custenGetTileConfig": (c_int, [_H, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                    ctypes.POINTER(c_int)]),
     "custenBatchCreate": (c_void_p, [c_int, c_int, _P, _P, _P, _P, _P, c_int, c_int, c_int,
                                      c_int, c_int]),  # noqa: E501
     "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
@@ -321,6 +323,13 @@ def custenSetForceIndirect(h: cuSten_t, force: bool) -> None:
 def custenSetTileConfig(h: cuSten_t, rowsPerWave: int, chunks: int, colsPerLane: int = 0) -> None:
     lib().custenSetTileConfigEx(_handle(h), rowsPerWave, chunks, colsPerLane)
     check("custenSetTileConfigEx")
+
+
+def custenGetTileConfig(h: cuSten_t):
+    r, c, l = c_int(), c_int(), c_int()
+    lib().custenGetTileConfig(_handle(h), ctypes.byref(r), ctypes.byref(c), ctypes.byref(l))
+    check("custenGetTileConfig")
+    return r.value, c.value, l.value
 
 
 def custenSetStream(h: cuSten_t, stream: int) -> None:

@@ -650,9 +650,11 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             k.ny = h->ny;
             TuneVal v;
             if (!tunerLookup(k, v)) {
-                static const std::vector<TuneVal> cands = {
-                    {4, 1, 2}, {4, 2, 2}, {4, 4, 2}, {8, 1, 2},  {8, 2, 2},  {8, 4, 2},
-                    {8, 8, 2}, {8, 2, 1}, {8, 4, 1}, {16, 1, 1}, {16, 2, 1}, {16, 4, 1}};
+                // one-column-per-lane strips (custenSetTileConfigEx) measured
+                // slower on every shape, so they are not candidates
+                static const std::vector<TuneVal> cands = {{4, 1, 2}, {4, 2, 2}, {4, 4, 2},
+                                                           {8, 1, 2}, {8, 2, 2}, {8, 4, 2},
+                                                           {8, 8, 2}, {8, 16, 2}};
                 auto run = [&](const TuneVal& c) {
                     Plan tmp = *p;
                     tmp.rowsPerWave = c.rpw;
@@ -660,7 +662,7 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
                     tmp.colsPerLane = c.cpl;
                     return launchRegion(h, &tmp, r, params, st);
                 };
-                e = tunerMeasure(st, run, cands, 3, v);
+                e = tunerMeasure(st, run, cands, 4, v);
                 if (e != hipSuccess) {
                     setHipError(e, fn);
                     return;
@@ -1024,6 +1026,17 @@ void custenSetTileConfigEx(cuSten_t* h, int rowsPerWave, int chunksPerGroup, int
 void custenSetTileConfig(cuSten_t* h, int rowsPerWave, int chunksPerGroup)
 {
     custenSetTileConfigEx(h, rowsPerWave, chunksPerGroup, 0);
+}
+
+int custenGetTileConfig(cuSten_t* h, int* rowsPerWave, int* chunksPerGroup, int* columnsPerLane)
+{
+    Plan* p = planOf(h, "custenGetTileConfig");
+    if (!p) return CUSTEN_ERROR_INVALID_VALUE;
+    const bool user = p->rowsPerWave || p->chunks || p->colsPerLane;
+    if (rowsPerWave) *rowsPerWave = user ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
+    if (chunksPerGroup) *chunksPerGroup = user ? p->chunks : (p->tuned ? p->tunedChunks : 0);
+    if (columnsPerLane) *columnsPerLane = user ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
+    return CUSTEN_SUCCESS;
 }
 
 void custenSetForceIndirect(cuSten_t* h, int force)

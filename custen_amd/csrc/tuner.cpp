@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -93,20 +94,36 @@ hipError_t tunerMeasure(hipStream_t st, const std::function<hipError_t(const Tun
         (void)hipEventDestroy(e0);
         return e;
     }
-    float bestMs = 1e30f;
-    best = cands.front();
-    for (const TuneVal& c : cands) {
-        if ((e = run(c)) != hipSuccess) break;  // warm-up
-        if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
-        for (int r = 0; r < reps && e == hipSuccess; r++) e = run(c);
-        if (e != hipSuccess) break;
-        if ((e = hipEventRecord(e1, st)) != hipSuccess) break;
-        if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+    // one pass over all candidates, then the three fastest again with twice
+    // the repetitions; the fastest of the second pass wins
+    auto timeOne = [&](const TuneVal& c, int n, float& ms) -> hipError_t {
+        hipError_t err;
+        if ((err = run(c)) != hipSuccess) return err;  // warm-up
+        if ((err = hipEventRecord(e0, st)) != hipSuccess) return err;
+        for (int r = 0; r < n; r++)
+            if ((err = run(c)) != hipSuccess) return err;
+        if ((err = hipEventRecord(e1, st)) != hipSuccess) return err;
+        if ((err = hipEventSynchronize(e1)) != hipSuccess) return err;
+        return hipEventElapsedTime(&ms, e0, e1);
+    };
+    std::vector<std::pair<float, size_t>> first;
+    e = hipSuccess;
+    for (size_t i = 0; i < cands.size() && e == hipSuccess; i++) {
         float ms = 0.f;
-        if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-        if (ms < bestMs) {
-            bestMs = ms;
-            best = c;
+        e = timeOne(cands[i], reps, ms);
+        first.push_back({ms, i});
+    }
+    best = cands.front();
+    if (e == hipSuccess) {
+        std::sort(first.begin(), first.end());
+        float bestMs = 1e30f;
+        for (size_t k = 0; k < first.size() && k < 3 && e == hipSuccess; k++) {
+            float ms = 0.f;
+            e = timeOne(cands[first[k].second], 2 * reps, ms);
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = cands[first[k].second];
+            }
         }
     }
     (void)hipEventDestroy(e0);

@@ -285,6 +285,10 @@ void custenSetStream(cuSten_t* pt_cuSten, hipStream_t stream);
 void custenSetTileConfig(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGroup);
 void custenSetTileConfigEx(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGroup,
                            int columnsPerLane);
+/* The configuration Compute uses: the caller's, else the autotuner's (all 0
+ * before the first tuned Compute). */
+int custenGetTileConfig(cuSten_t* pt_cuSten, int* rowsPerWave, int* chunksPerGroup,
+                        int* columnsPerLane);
 
 /* Fill out[0..n) with the SplitMix64 uniform stream (value i from state
  * seed + (i+1)*0x9E3779B97F4A7C15, u = (z >> 11) 2^-53, a + (b - a) u). Host. */
