@@ -39,20 +39,43 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
 
     const int y00 = tyg * a.chunks * TH;
     if (y00 >= a.ny) return;
-    StagedRows<T, NR, CPL> st;
-    loadRows<T, NR, FAST, CPL>(st, in, a.ldIn, a.rows, y00, a.T, x0, nx, L, R, a.xWrap, vecIn,
-                               wave, lane);
-
+    const int B = V - 1 - a.T;
+    // Chunk c needs region rows [y0 - T, y0 + TH + B).  The first chunk loads
+    // all NR of them; every later chunk keeps its first V-1 rows from the
+    // previous tile (LDS rows [TH, NR) move to [0, V-1)) and loads only the
+    // TH new rows [y0 + B, y0 + TH + B), prefetched while the previous chunk
+    // computes.
+    {
+        StagedRows<T, NR, CPL> first;
+        loadRows<T, NR, FAST, CPL>(first, in, a.ldIn, a.rows, y00, a.T, x0, nx, L, R, a.xWrap,
+                                   vecIn, wave, lane);
+        storeRows<T, X, NR, CPL>(first, tile, LD, L, R, LP, wave, lane);
+    }
+    constexpr int NC = ((V - 1) * LD + kThreads - 1) / kThreads;  // carried values per thread
+    StagedRows<T, TH, CPL> st;
     for (int c = 0; c < a.chunks; c++) {
         const int y0 = y00 + c * TH;
-        if (c > 0) __syncthreads();  // the previous chunk's compute is done with LDS
-        storeRows<T, X, NR, CPL>(st, tile, LD, L, R, LP, wave, lane);
+        if (c > 0) {
+            T carry[NC > 0 ? NC : 1];
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const int idx = threadIdx.x + k * kThreads;
+                if (idx < (V - 1) * LD) carry[k] = tile[TH * LD + idx];
+            }
+            __syncthreads();  // every wave is done with the previous chunk
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const int idx = threadIdx.x + k * kThreads;
+                if (idx < (V - 1) * LD) tile[idx] = carry[k];
+            }
+            storeRows<T, X, TH, CPL>(st, tile + (V - 1) * LD, LD, L, R, LP, wave, lane);
+        }
         __syncthreads();
         const bool more = (c + 1 < a.chunks) && (y0 + TH < a.ny);
-        // prefetch the next chunk's rows; they stay in flight during the compute
+        // prefetch the next chunk's new rows; they stay in flight during the compute
         if (more)
-            loadRows<T, NR, FAST, CPL>(st, in, a.ldIn, a.rows, y0 + TH, a.T, x0, nx, L, R, a.xWrap,
-                                       vecIn, wave, lane);
+            loadRows<T, TH, FAST, CPL>(st, in, a.ldIn, a.rows, y0 + TH + B, 0, x0, nx, L, R,
+                                       a.xWrap, vecIn, wave, lane);
 
         T acc[RPW][CPL];
 #pragma unroll

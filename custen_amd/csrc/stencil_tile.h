@@ -154,7 +154,10 @@ template <typename T, int NR, int CPL = 2>
 struct StagedRows {
     static constexpr int PER = (NR + kWaves - 1) / kWaves;  // rows per wave
     typename LaneVec<T, CPL>::type body[PER];
-    T halo[PER];
+    // halo columns of the wave's PER rows, one value per lane: lane
+    // i * (L + R) + k holds column k of the wave's i-th row (PER * (L + R)
+    // values; a second register covers up to 128 of them)
+    T halo[2];
 };
 
 // Source row of region row gy when the region is at least as tall as the
@@ -194,15 +197,27 @@ __device__ __forceinline__ void loadRows(StagedRows<T, NR, CPL>& s, const T* __r
 #pragma unroll
                 for (int e = 0; e < CPL; e++) LV::set(s.body[i], e, grow[mapCol(gx + e, nx, xWrap)]);
             }
-            if (lane < L + R) {
-                int hx = lane < L ? x0 - L + lane : x0 + TW + (lane - L);
+        }
+    }
+    // halo columns: lane q -> (row i, column k) with q = i * (L + R) + k
+    const int HW = L + R;
+    if (HW > 0) {
+#pragma unroll
+        for (int part = 0; part < 2; part++) {
+            const int q = lane + part * kLanes;
+            const int i = q / HW;
+            const int k = q - i * HW;
+            const int r = wave + i * kWaves;
+            if (i < StagedRows<T, NR, CPL>::PER && r < NR) {
+                const int sy = FAST ? mapRowNear(rows, y0 - T_ + r) : mapRow(rows, y0 - T_ + r);
+                int hx = k < L ? x0 - L + k : x0 + TW + (k - L);
                 if (FAST) {
                     if (xWrap) hx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx);
                     else hx = hx < 0 ? 0 : (hx >= nx ? nx - 1 : hx);
                 } else {
                     hx = mapCol(hx, nx, xWrap);
                 }
-                s.halo[i] = grow[hx];
+                s.halo[part] = in[(int64_t)sy * ldIn + hx];
             }
         }
     }
@@ -224,9 +239,19 @@ __device__ __forceinline__ void storeRows(const StagedRows<T, NR, CPL>& s, T* __
 #pragma unroll
             for (int e = 0; e < CPL; e++) LV::set(o, e, xformValue<T, X>(LV::get(s.body[i], e)));
             *reinterpret_cast<VT*>(lrow + LP + CPL * lane) = o;
-            if (lane < L + R) {
-                const int hx = lane < L ? LP - L + lane : LP + TW + (lane - L);
-                lrow[hx] = xformValue<T, X>(s.halo[i]);
+        }
+    }
+    const int HW = L + R;
+    if (HW > 0) {
+#pragma unroll
+        for (int part = 0; part < 2; part++) {
+            const int q = lane + part * kLanes;
+            const int i = q / HW;
+            const int k = q - i * HW;
+            const int r = wave + i * kWaves;
+            if (i < StagedRows<T, NR, CPL>::PER && r < NR) {
+                const int hx = k < L ? LP - L + k : LP + TW + (k - L);
+                tile[r * LD + hx] = xformValue<T, X>(s.halo[part]);
             }
         }
     }

@@ -106,8 +106,18 @@ hipError_t tunerMeasure(hipStream_t st, const std::function<hipError_t(const Tun
         if ((err = hipEventSynchronize(e1)) != hipSuccess) return err;
         return hipEventElapsedTime(&ms, e0, e1);
     };
+    // Short kernels: repeat until each timing covers about 4 ms, after about
+    // 20 ms of warm-up work, so that clock ramp-up does not favour the
+    // candidates measured last.
+    float probe = 0.f;
+    e = timeOne(cands.front(), reps, probe);
+    if (e == hipSuccess && probe > 0.f) {
+        const float perRep = probe / reps;
+        const int warm = std::min(2000, static_cast<int>(20.f / perRep) + 1);
+        e = timeOne(cands.front(), warm, probe);
+        reps = std::max(reps, std::min(400, static_cast<int>(4.f / perRep) + 1));
+    }
     std::vector<std::pair<float, size_t>> first;
-    e = hipSuccess;
     for (size_t i = 0; i < cands.size() && e == hipSuccess; i++) {
         float ms = 0.f;
         e = timeOne(cands[i], reps, ms);

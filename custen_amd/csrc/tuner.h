@@ -27,8 +27,9 @@ struct TuneVal {
 bool tunerEnabled();
 bool tunerLookup(const TuneKey& k, TuneVal& v);
 void tunerStore(const TuneKey& k, const TuneVal& v);
-// Runs every candidate once untimed and `reps` times timed on `st`; the
-// fastest lands in `best`.
+// Times every candidate on `st` (at least `reps` launches each, more for
+// short kernels, after a warm-up), re-times the three fastest, and leaves the
+// fastest in `best`.
 hipError_t tunerMeasure(hipStream_t st, const std::function<hipError_t(const TuneVal&)>& run,
                         const std::vector<TuneVal>& cands, int reps, TuneVal& best);
 bool streamCapturing(hipStream_t st);
