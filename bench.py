@@ -196,12 +196,69 @@ def bench_stencil_8192(args, torch, api, extra):
                                   "max_err_rel_to_max_abs_output": err_abs / scale}
     del din32, dout32
 
+    bench_weno_8192(torch, api, stream, timer, din, dout, k2, extra)
+
     # measured copy ceiling: device-to-device copy of the same 512 MiB
     tc = timer.run(lambda: dout.copy_(din), k2, 2)
     extra["copy_ceiling_GBps"] = round(2 * 8.0 * pts / (tc * 1e-3) / 1e9, 1)
     del din, dout
     torch.cuda.empty_cache()
     return ms
+
+
+def bench_weno_8192(torch, api, stream, timer, din, dout, steps, extra):
+    """WENO5 advection of the 8192^2 field by synthetic u, v (32 B/pt: read the
+    field, u and v, write the result), both treatments of the smoothness
+    squares and both tile heights; a 256-row band is checked against the
+    oracle on a periodic 8192 x 256 slab of its own."""
+    import oracle
+    from custen_amd import synth
+
+    n = din.shape[0]
+    pts = n * n
+    hu = np.empty(pts)
+    hv = np.empty(pts)
+    api.custenFillUniform(hu, pts, 6, -1.0, 1.0)
+    api.custenFillUniform(hv, pts, 7, -1.0, 1.0)
+    du = torch.from_numpy(hu.reshape(n, n)).cuda()
+    dv = torch.from_numpy(hv.reshape(n, n)).cuda()
+    del hu, hv
+    dx = 2 * np.pi / n
+    res = {}
+    for single in (True, False):
+        for rpw in (4, 8):
+            h = api.cuSten_t()
+            api.cuStenCreate2DXYWENOADVp(h, 0, 1, n, n, 32, 32, dx, dx, du, dv, dout, din)
+            api.custenSetStream(h, stream.cuda_stream)
+            api.custenSetWenoSquares(h, single)
+            api.custenSetTileConfig(h, rpw, 1)
+            fn = lambda: api.cuStenCompute2DXYWENOADVp(h, api.DEVICE)  # noqa: E731
+            fn()
+            torch.cuda.synchronize()
+            t = timer.run(fn, steps, 2)
+            api.cuStenDestroy2DXYWENOADVp(h)
+            res[f"{'single' if single else 'double'}_squares_rpw{rpw}"] = {
+                "us": round(t * 1e3, 1), "GPts/s": round(pts / (t * 1e-3) / 1e9, 1),
+                "GB/s": round(32.0 * pts / (t * 1e-3) / 1e9, 1)}
+    # parity on a slab: a separate periodic 8192 x 256 problem
+    rows = 256
+    phi = synth.uniform(n * rows, 8, -1.0, 1.0).reshape(rows, n)
+    us = du[:rows].cpu().numpy().copy()
+    vs = dv[:rows].cpu().numpy().copy()
+    sphi, sout = torch.from_numpy(phi).cuda(), torch.empty(rows, n, dtype=torch.float64,
+                                                           device="cuda")
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYWENOADVp(h, 0, 1, n, rows, 32, 32, dx, dx, du[:rows], dv[:rows], sout,
+                                 sphi)
+    api.cuStenCompute2DXYWENOADVp(h, api.DEVICE)
+    torch.cuda.synchronize()
+    api.cuStenDestroy2DXYWENOADVp(h)
+    ref = oracle.weno(phi, us, vs, dx, dx, True, nthreads=min(16, os.cpu_count() or 1))
+    res["slab_mismatching_points_vs_oracle"] = int(np.count_nonzero(
+        sout.cpu().numpy().view(np.uint64) != ref.view(np.uint64)))
+    extra["xy_p_weno_8192"] = res
+    log(f"weno 8192^2: {res}")
+    del du, dv, sphi, sout
 
 
 def bench_x_1024(args, torch, api, extra):

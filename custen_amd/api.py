@@ -129,7 +129,14 @@ for _v, _cargs in _VARIANTS.items():
         _SIGS[f"{_pre}Compute{_v}"] = (None, [_H, c_bool])
         _SIGS[f"{_pre}Swap{_v}"] = (None, [_H, _P])
         _SIGS[f"{_pre}Destroy{_v}"] = (None, [_H])
+_WENO_CREATE = [_H, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double, _P, _P, _P, _P]
+for _pre in ("cuSten", "custen"):
+    _SIGS[f"{_pre}Create2DXYWENOADVp"] = (None, _WENO_CREATE)
+    _SIGS[f"{_pre}Compute2DXYWENOADVp"] = (None, [_H, c_bool])
+    _SIGS[f"{_pre}Swap2DXYWENOADVp"] = (None, [_H, _P])
+    _SIGS[f"{_pre}Destroy2DXYWENOADVp"] = (None, [_H])
 _SIGS.update({
+    "custenSetWenoSquares": (None, [_H, c_int]),
     "cuSenCompute2DXpFun": (None, [_H, c_bool]),
     "checkError": (None, [c_char_p]),
     "custenGetLastError": (c_int, []),
@@ -293,6 +300,30 @@ for _v in _VARIANTS:
         globals()[f"{_pre}Swap{_v}"] = _make_swap(f"{_pre}Swap{_v}")
         globals()[f"{_pre}Destroy{_v}"] = _make_destroy(f"{_pre}Destroy{_v}")
 cuSenCompute2DXpFun = _make_compute("cuSenCompute2DXpFun")
+
+
+def _make_weno_create(name):
+    def create(h: cuSten_t, deviceNum, numTiles, nx, ny, BLOCK_X, BLOCK_Y, dx, dy, u, v,
+               dataOutput, dataInput):
+        getattr(lib(), name)(_handle(h), deviceNum, numTiles, nx, ny, BLOCK_X, BLOCK_Y,
+                             float(dx), float(dy), _addr(u), _addr(v), _addr(dataOutput),
+                             _addr(dataInput))
+        check(name)
+
+    create.__name__ = name
+    return create
+
+
+for _pre in ("cuSten", "custen"):
+    globals()[f"{_pre}Create2DXYWENOADVp"] = _make_weno_create(f"{_pre}Create2DXYWENOADVp")
+    globals()[f"{_pre}Compute2DXYWENOADVp"] = _make_compute(f"{_pre}Compute2DXYWENOADVp")
+    globals()[f"{_pre}Swap2DXYWENOADVp"] = _make_swap(f"{_pre}Swap2DXYWENOADVp")
+    globals()[f"{_pre}Destroy2DXYWENOADVp"] = _make_destroy(f"{_pre}Destroy2DXYWENOADVp")
+
+
+def custenSetWenoSquares(h: cuSten_t, singlePrecision: bool) -> None:
+    lib().custenSetWenoSquares(_handle(h), int(bool(singlePrecision)))
+    check("custenSetWenoSquares")
 
 
 def custenCreate2DF32(h, deviceNum, numTiles, nx, ny, dir_, periodic, dataOutput, dataInput,

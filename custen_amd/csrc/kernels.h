@@ -89,18 +89,23 @@ hipError_t builtinFunctionAddress(Builtin b, void** addr);
 // LDS bytes the stencil kernel uses for a given shape (reported in mem_shared).
 int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave);
 
-// Batched launch over many grids (periodic xy, shared weights).
-struct BatchGridDesc {
-    const void* in;
-    void* out;
+// WENO5 upwind advection over a region of rows [0, ny), periodic in x: out =
+// u * Fx + v * Fy.  `in` rows are addressed through `rows` (the kernel reads
+// region rows -3 .. ny + 2); u, v and out are region-local.
+struct WenoLaunch {
+    const double* in;
+    const double* u;
+    const double* v;
+    double* out;
+    int64_t ldIn, ldSide, ldOut;  // row strides of in, of u and v, of out
     int nx, ny;
-    int tilesX;              // tiles across x
-    int tileBase;            // first global tile id of this grid
+    RowMap rows;
+    double coeDx, coeDy;          // 1/dx, 1/dy
+    int singleSquares;            // 1: smoothness squares in single precision
+    int rowsPerWave;              // 4 or 8 (0 = default)
 };
-hipError_t launchBatch(Dtype dt, const BatchGridDesc* grids, int numGrids, int totalTiles,
-                       const double* weights, int H, int L, int V, int T, int rowsPerWave,
-                       hipStream_t stream);
-int batchTileWidth();
-int batchTileHeight(int rowsPerWave);
+
+hipError_t launchWeno(const WenoLaunch& w, hipStream_t stream);
+int wenoLdsBytes(int rowsPerWave);
 
 }  // namespace custen

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -110,6 +111,10 @@ struct CreateArgs {
     int numStenHoriz, L, R, numStenVert, T, B;
     int numCoe;       // Fun only
     double* func;     // Fun only
+    bool weno;        // WENO advection: u, v velocities, dx, dy spacings
+    double* u;
+    double* v;
+    double dx, dy;
 };
 
 size_t elemSize(Dtype d) { return d == Dtype::F64 ? sizeof(double) : sizeof(float); }
@@ -148,6 +153,7 @@ void releaseHandle(cuSten_t* h, Plan* p)
         for (int b = 0; b < 2; b++) {
             if (p->stageIn[b]) (void)hipFree(p->stageIn[b]);
             if (p->stageOut[b]) (void)hipFree(p->stageOut[b]);
+            if (p->stageSide[b]) (void)hipFree(p->stageSide[b]);
             if (p->evH2D[b]) (void)hipEventDestroy(p->evH2D[b]);
             if (p->evComp[b]) (void)hipEventDestroy(p->evComp[b]);
             if (p->evD2H[b]) (void)hipEventDestroy(p->evD2H[b]);
@@ -170,9 +176,12 @@ void releaseHandle(cuSten_t* h, Plan* p)
     std::free(h->dataOutput);
     std::free(h->boundaryTop);
     std::free(h->boundaryBottom);
+    std::free(h->uVel);
+    std::free(h->vVel);
     h->streams = nullptr;
     h->events = nullptr;
     h->dataInput = h->dataOutput = h->boundaryTop = h->boundaryBottom = nullptr;
+    h->uVel = h->vVel = nullptr;
     h->plan = nullptr;
 }
 
@@ -196,7 +205,12 @@ bool validate(const CreateArgs& a, std::string& why)
     if (a.numStenHoriz > 65 || a.numStenVert > 65) return bad("stencil extent above 65 points");
     if (a.in == nullptr || a.out == nullptr) return bad("null data pointer");
     if (a.in == a.out) return bad("input and output must not alias");
-    if (a.fun) {
+    if (a.weno) {
+        if (a.u == nullptr || a.v == nullptr) return bad("null velocity pointer");
+        if (a.u == a.out || a.v == a.out) return bad("velocities must not alias the output");
+        if (!(std::isfinite(a.dx) && std::isfinite(a.dy)) || a.dx == 0.0 || a.dy == 0.0)
+            return bad("dx and dy must be finite and nonzero");
+    } else if (a.fun) {
         if (a.func == nullptr) return bad("null device function pointer");
         if (a.numCoe < 0 || a.numCoe > 1024) return bad("numCoe must be in [0, 1024]");
         if (a.numCoe > 0 && a.weights == nullptr) return bad("null coefficient pointer");
@@ -287,8 +301,14 @@ void createCommon(cuSten_t* h, const CreateArgs& a)
     h->yGrid = (h->nyTile + a.BY - 1) / a.BY;
     h->nxLocal = (a.dir == Dir::Y) ? a.BX : a.BX + a.L + a.R;
     h->nyLocal = (a.dir == Dir::X) ? a.BY : a.BY + a.T + a.B;
-    h->mem_shared = stencilLdsBytes(a.dtype, p->H, p->V, 0);
-    if (a.fun) {
+    h->mem_shared = a.weno ? wenoLdsBytes(0) : stencilLdsBytes(a.dtype, p->H, p->V, 0);
+    if (a.weno) {
+        // custenCreateDestroy2DXYADVWENOp.cu:99-117: coefficients 1/dx, 1/dy,
+        // a 7 x 7 window (3 points each side)
+        p->weno = true;
+        h->coeDx = 1.0 / a.dx;
+        h->coeDy = 1.0 / a.dy;
+    } else if (a.fun) {
         h->coe = a.weights;
         h->numCoe = a.numCoe;
         h->devFunc = a.func;
@@ -305,8 +325,13 @@ void createCommon(cuSten_t* h, const CreateArgs& a)
         h->boundaryTop = static_cast<double**>(std::calloc(a.numTiles, sizeof(double*)));
         h->boundaryBottom = static_cast<double**>(std::calloc(a.numTiles, sizeof(double*)));
     }
+    if (a.weno) {
+        h->uVel = static_cast<double**>(std::calloc(a.numTiles, sizeof(double*)));
+        h->vVel = static_cast<double**>(std::calloc(a.numTiles, sizeof(double*)));
+    }
     if (!h->streams || !h->events || !h->dataInput || !h->dataOutput ||
-        (halos && (!h->boundaryTop || !h->boundaryBottom))) {
+        (halos && (!h->boundaryTop || !h->boundaryBottom)) ||
+        (a.weno && (!h->uVel || !h->vVel))) {
         releaseHandle(h, p);
         setError(CUSTEN_ERROR_OUT_OF_MEMORY, std::string(a.fn) + ": out of host memory");
         return;
@@ -331,6 +356,10 @@ void createCommon(cuSten_t* h, const CreateArgs& a)
     for (int t = 0; t < a.numTiles; t++) {
         h->dataInput[t] = offsetPtr(a.in, t * tileElems, a.dtype);
         h->dataOutput[t] = offsetPtr(a.out, t * tileElems, a.dtype);
+        if (a.weno) {
+            h->uVel[t] = a.u + t * tileElems;
+            h->vVel[t] = a.v + t * tileElems;
+        }
     }
     if (halos) {
         h->numBoundaryTop = a.T * a.nx;
@@ -376,6 +405,8 @@ struct Region {
     int nx, ny;
     RowMap rows;
     int yGlobal0;
+    const double* u = nullptr;  // WENO velocities, region-local, row stride ldOut
+    const double* v = nullptr;
 };
 
 hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const double* params,
@@ -387,6 +418,24 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
     const int wx1 = p->periodic ? h->nx : h->nx - R;
     const int wy0 = p->periodic ? 0 : p->T;
     const int wy1 = p->periodic ? h->ny : h->ny - B;
+    if (p->weno) {
+        WenoLaunch w{};
+        w.in = static_cast<const double*>(r.in);
+        w.u = r.u;
+        w.v = r.v;
+        w.out = static_cast<double*>(r.out);
+        w.ldIn = r.ldIn;
+        w.ldSide = r.ldOut;
+        w.ldOut = r.ldOut;
+        w.nx = r.nx;
+        w.ny = r.ny;
+        w.rows = r.rows;
+        w.coeDx = h->coeDx;
+        w.coeDy = h->coeDy;
+        w.singleSquares = p->wenoSingleSquares ? 1 : 0;
+        w.rowsPerWave = p->rowsPerWave;
+        return launchWeno(w, st);
+    }
     if (!p->fun) {
         StencilLaunch s{};
         s.dtype = p->dtype;
@@ -451,6 +500,9 @@ bool tablesConsistent(const cuSten_t* h, const Plan* p)
         if (h->dataInput[t] != offsetPtr(h->dataInput[0], t * tileElems, p->dtype)) return false;
         if (h->dataOutput[t] != offsetPtr(h->dataOutput[0], t * tileElems, p->dtype))
             return false;
+        if (p->weno && (h->uVel[t] != h->uVel[0] + t * tileElems ||
+                        h->vVel[t] != h->vVel[0] + t * tileElems))
+            return false;
     }
     return true;
 }
@@ -499,6 +551,23 @@ void computeStreamed(cuSten_t* h, Plan* p, const double* params, const char* fn)
         p->stageInBytes = inBytes;
         p->stageOutBytes = outBytes;
     }
+    // WENO: u and v of the tile's own rows (no halo) share one slot buffer
+    const size_t sideBytes = p->weno ? 2 * outBytes : 0;
+    if (p->stageSideBytes < sideBytes) {
+        for (int b = 0; b < 2; b++) {
+            if (p->stageSide[b]) (void)hipFree(p->stageSide[b]);
+            p->stageSide[b] = nullptr;
+        }
+        p->stageSideBytes = 0;
+        const int slots = h->numTiles > 1 ? 2 : 1;
+        for (int b = 0; b < slots; b++) {
+            if ((e = hipMalloc(&p->stageSide[b], sideBytes)) != hipSuccess) {
+                setHipError(e, fn);
+                return;
+            }
+        }
+        p->stageSideBytes = sideBytes;
+    }
     const char* inHost = reinterpret_cast<const char*>(h->dataInput[0]);
     char* outHost = reinterpret_cast<char*>(h->dataOutput[0]);
     hipStream_t sComp = h->streams[0], sIn = h->streams[1], sOut = h->streams[2];
@@ -544,6 +613,15 @@ void computeStreamed(cuSten_t* h, Plan* p, const double* params, const char* fn)
             k += run;
         }
         if (e != hipSuccess) break;
+        if (p->weno) {
+            char* side = static_cast<char*>(p->stageSide[b]);
+            const size_t off = (size_t)((int64_t)t * nyT * nx) * es;
+            if ((e = hipMemcpyAsync(side, reinterpret_cast<const char*>(h->uVel[0]) + off,
+                                    outBytes, hipMemcpyDefault, sIn)) != hipSuccess ||
+                (e = hipMemcpyAsync(side + outBytes, reinterpret_cast<const char*>(h->vVel[0]) + off,
+                                    outBytes, hipMemcpyDefault, sIn)) != hipSuccess)
+                break;
+        }
         if ((e = hipEventRecord(p->evH2D[b], sIn)) != hipSuccess) break;
         if ((e = hipStreamWaitEvent(sComp, p->evH2D[b], 0)) != hipSuccess) break;
         if (t >= slots) {
@@ -561,6 +639,10 @@ void computeStreamed(cuSten_t* h, Plan* p, const double* params, const char* fn)
         r.rows.rowMin = -T;
         r.rows.rowMax = (int)(nyT + B - 1);
         r.yGlobal0 = (int)((int64_t)t * nyT);
+        if (p->weno) {
+            r.u = static_cast<const double*>(p->stageSide[b]);
+            r.v = r.u + nyT * nx;
+        }
         if ((e = launchRegion(h, p, r, params, sComp)) != hipSuccess) break;
         if ((e = hipEventRecord(p->evComp[b], sComp)) != hipSuccess) break;
         if ((e = hipStreamWaitEvent(sOut, p->evComp[b], 0)) != hipSuccess) break;
@@ -607,10 +689,12 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
     const void* in = h->dataInput[0];
     void* out = h->dataOutput[0];
     const bool resident = classify(in, h->deviceNum) == MemKind::Device &&
-                          classify(out, h->deviceNum) == MemKind::Device;
+                          classify(out, h->deviceNum) == MemKind::Device &&
+                          (!p->weno || (classify(h->uVel[0], h->deviceNum) == MemKind::Device &&
+                                        classify(h->vVel[0], h->deviceNum) == MemKind::Device));
     hipStream_t st = p->hasUserStream ? p->userStream : h->streams[0];
-    const double* src = p->fun ? h->coe : h->weights;
-    const int count = p->fun ? p->numCoe : p->H * p->V;
+    const double* src = p->weno ? nullptr : p->fun ? h->coe : h->weights;
+    const int count = p->weno ? 0 : p->fun ? p->numCoe : p->H * p->V;
     const double* params = deviceParams(p, src, count, h->deviceNum,
                                         resident ? st : h->streams[1], fn);
     if (src != nullptr && count > 0 && params == nullptr) return;
@@ -634,9 +718,14 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             r.rows.rowMax = h->ny - 1;
         }
         r.yGlobal0 = 0;
+        if (p->weno) {
+            r.u = h->uVel[0];
+            r.v = h->vVel[0];
+        }
         // First resident Compute of a large grid: pick the tile configuration
         // (bit-identical results for every candidate; synchronous once).
-        if (!p->tuned && p->rowsPerWave == 0 && p->chunks == 0 && p->colsPerLane == 0 &&
+        if (!p->weno && !p->tuned && p->rowsPerWave == 0 && p->chunks == 0 &&
+            p->colsPerLane == 0 &&
             tunerEnabled() &&
             (int64_t)h->nx * h->ny >= (int64_t(1) << 20) && !streamCapturing(st)) {
             TuneKey k;
@@ -959,6 +1048,41 @@ void cuStenCreate2DXYpFun(cuSten_t* h, int deviceNum, int numTiles, int nx, int 
 void cuStenSwap2DXYpFun(cuSten_t* h, double* dataInput) { swapCommon(h, dataInput, "cuStenSwap2DXYpFun"); }
 void cuStenDestroy2DXYpFun(cuSten_t* h) { destroyCommon(h, "cuStenDestroy2DXYpFun"); }
 void cuStenCompute2DXYpFun(cuSten_t* h, bool offload) { computeCommon(h, offload, "cuStenCompute2DXYpFun"); }
+
+// ---- WENO5 advection (custenCreateDestroy2DXYADVWENOp.cu:57-260) ---------------
+void cuStenCreate2DXYWENOADVp(cuSten_t* h, int deviceNum, int numTiles, int nx, int ny,
+                              int BLOCK_X, int BLOCK_Y, double dx, double dy, double* u, double* v,
+                              double* dataOutput, double* dataInput)
+{
+    CreateArgs a = baseArgs("cuStenCreate2DXYWENOADVp", Dir::XY, true, false, deviceNum, numTiles,
+                            nx, ny, BLOCK_X, BLOCK_Y, dataOutput, dataInput, nullptr);
+    a.numStenHoriz = 7;
+    a.L = 3;
+    a.R = 3;
+    a.numStenVert = 7;
+    a.T = 3;
+    a.B = 3;
+    a.weno = true;
+    a.u = u;
+    a.v = v;
+    a.dx = dx;
+    a.dy = dy;
+    createCommon(h, a);
+}
+void cuStenSwap2DXYWENOADVp(cuSten_t* h, double* dataInput) { swapCommon(h, dataInput, "cuStenSwap2DXYWENOADVp"); }
+void cuStenDestroy2DXYWENOADVp(cuSten_t* h) { destroyCommon(h, "cuStenDestroy2DXYWENOADVp"); }
+void cuStenCompute2DXYWENOADVp(cuSten_t* h, bool offload) { computeCommon(h, offload, "cuStenCompute2DXYWENOADVp"); }
+
+void custenSetWenoSquares(cuSten_t* h, int singlePrecision)
+{
+    Plan* p = planOf(h, "custenSetWenoSquares");
+    if (!p) return;
+    if (!p->weno) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetWenoSquares: not a WENO handle");
+        return;
+    }
+    p->wenoSingleSquares = singlePrecision != 0;
+}
 
 // ---- FP32 storage -----------------------------------------------------------
 void custenCreate2DF32(cuSten_t* h, int deviceNum, int numTiles, int nx, int ny, int dir,

@@ -28,6 +28,8 @@ struct Plan {
     Builtin builtin = Builtin::None;
     bool forceIndirect = false;
     bool zeroRight = false;     // x-np: right R columns written as 0.0
+    bool weno = false;          // WENO5 advection (cuStenCreate2DXYWENOADVp)
+    bool wenoSingleSquares = true;  // smoothness squares in single precision (as powf)
     hipStream_t userStream = nullptr;
     bool hasUserStream = false;
     int rowsPerWave = 0;        // set by custenSetTileConfig[Ex] (0: autotuned)
@@ -41,7 +43,8 @@ struct Plan {
     // streaming resources (host-resident data)
     void* stageIn[2] = {nullptr, nullptr};
     void* stageOut[2] = {nullptr, nullptr};
-    size_t stageInBytes = 0, stageOutBytes = 0;
+    void* stageSide[2] = {nullptr, nullptr};  // WENO: u and v rows of a tile
+    size_t stageInBytes = 0, stageOutBytes = 0, stageSideBytes = 0;
     hipEvent_t evH2D[2] = {nullptr, nullptr};
     hipEvent_t evComp[2] = {nullptr, nullptr};
     hipEvent_t evD2H[2] = {nullptr, nullptr};

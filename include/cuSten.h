@@ -208,6 +208,21 @@ void cuStenSwap2DXYpFun(cuSten_t* pt_cuSten, double* dataInput);
 void cuStenDestroy2DXYpFun(cuSten_t* pt_cuSten);
 void cuStenCompute2DXYpFun(cuSten_t* pt_cuSten, bool offload);
 
+/* ---- 2D xy periodic WENO5 upwind advection ------------------------------- */
+/* cuSten_struct_functions.h:309-346, stencil_kernels.h:184; behaviour
+ * custenCreateDestroy2DXYADVWENOp.cu:57-260 and 2d_xyADVWENO_p_kernel.cu.
+ * dataOutput = u * d(dataInput)/dx + v * d(dataInput)/dy with fifth-order WENO
+ * fluxes upwinded on the signs of u and v; coeDx = 1/dx, coeDy = 1/dy.  u and
+ * v are nx*ny fields like the data.  The smoothness squares are taken in
+ * single precision by default, as the reference's powf calls on doubles do;
+ * custenSetWenoSquares(h, 0) takes them in double. */
+void cuStenCreate2DXYWENOADVp(cuSten_t* pt_cuSten, int deviceNum, int numTiles, int nx, int ny,
+                              int BLOCK_X, int BLOCK_Y, double dx, double dy, double* u,
+                              double* v, double* dataOutput, double* dataInput);
+void cuStenSwap2DXYWENOADVp(cuSten_t* pt_cuSten, double* dataInput);
+void cuStenDestroy2DXYWENOADVp(cuSten_t* pt_cuSten);
+void cuStenCompute2DXYWENOADVp(cuSten_t* pt_cuSten, bool offload);
+
 /* Error convention of the reference (util.h:43, error.cu:41-50): if the last
  * HIP call (or the last library call) failed, print `msg` with the error text
  * and exit(EXIT_FAILURE). */
@@ -290,6 +305,10 @@ void custenSetTileConfigEx(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGr
 int custenGetTileConfig(cuSten_t* pt_cuSten, int* rowsPerWave, int* chunksPerGroup,
                         int* columnsPerLane);
 
+/* WENO handles: 1 (default) squares the smoothness terms in single precision
+ * as the reference does, 0 squares them in double. */
+void custenSetWenoSquares(cuSten_t* pt_cuSten, int singlePrecision);
+
 /* Fill out[0..n) with the SplitMix64 uniform stream (value i from state
  * seed + (i+1)*0x9E3779B97F4A7C15, u = (z >> 11) 2^-53, a + (b - a) u). Host. */
 void custenFillUniform(double* out, size_t n, uint64_t seed, double a, double b);
@@ -311,7 +330,9 @@ void custenFillUniform(double* out, size_t n, uint64_t seed, double a, double b)
     X(Create2DXYnp) X(Swap2DXYnp) X(Destroy2DXYnp) X(Compute2DXYnp)              \
     X(Create2DXYnpFun) X(Swap2DXYnpFun) X(Destroy2DXYnpFun) X(Compute2DXYnpFun)  \
     X(Create2DXYp) X(Swap2DXYp) X(Destroy2DXYp) X(Compute2DXYp)                  \
-    X(Create2DXYpFun) X(Swap2DXYpFun) X(Destroy2DXYpFun) X(Compute2DXYpFun)
+    X(Create2DXYpFun) X(Swap2DXYpFun) X(Destroy2DXYpFun) X(Compute2DXYpFun)      \
+    X(Create2DXYWENOADVp) X(Swap2DXYWENOADVp) X(Destroy2DXYWENOADVp)             \
+    X(Compute2DXYWENOADVp)
 
 #ifndef CUSTEN_NO_LOWERCASE_ALIASES
 #ifdef __cplusplus

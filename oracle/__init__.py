@@ -64,6 +64,8 @@ def lib():
         L.cso_pent_solve.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _i]
         L.cso_fill_uniform.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_double,
                                        ctypes.c_double]
+        L.cso_weno_f64.argtypes = [_dp, _dp, _dp, _dp, _l, _l, ctypes.c_double, ctypes.c_double,
+                                   _i, _i]
         _lib = L
     return _lib
 
@@ -103,6 +105,22 @@ def apply_fun(fun_id, direction, inp, coe, H, L, V, T, periodic, out=None, nthre
     coe = np.ascontiguousarray(coe, dtype=np.float64)
     lib().cso_apply_fun_f64(fun_id, direction, _d(inp), _d(out), nx, ny, _d(coe), coe.size, H, L,
                             V, T, int(periodic), nthreads)
+    return out
+
+
+def weno(inp, u, v, dx, dy, single_squares=True, out=None, nthreads=1):
+    """WENO5 upwind advection u * d/dx + v * d/dy of a periodic (ny, nx) field
+    (2d_xyADVWENO_p_kernel.cu:51-401).  coeDx = 1/dx and coeDy = 1/dy, as the
+    reference's Create computes them."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    ny, nx = inp.shape
+    assert u.shape == inp.shape and v.shape == inp.shape
+    if out is None:
+        out = np.zeros_like(inp)
+    lib().cso_weno_f64(_d(inp), _d(u), _d(v), _d(out), nx, ny, 1.0 / dx, 1.0 / dy,
+                       int(single_squares), nthreads)
     return out
 
 

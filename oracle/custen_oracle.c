@@ -194,6 +194,75 @@ void cso_apply_fun_f64(int fun_id, int dir, const double* in, double* out,
 }
 
 /* --------------------------------------------------------------------------
+ * WENO5 upwind advection, xy periodic.
+ *
+ * Restates cuSten/src/kernels/2d_xyADVWENO_p_kernel.cu:
+ *   wenoSten :51-86 (Fedkiw's notation: three candidate stencils phi1..3,
+ *            smoothness s1..s3 with the squares taken by powf on doubles,
+ *            weights 0.1 / 0.6 / 0.3 over (s + 1e-6)^2, normalised by a
+ *            reciprocal), differences (q[k] - q[k-1]) * coeD, upwind choice on
+ *            u > 0 (else the mirrored five differences) :284-346 for x and
+ *            :348-397 for y, output u * Fx + v * Fy :401.
+ * Every operation is a separate IEEE double operation in source order; the
+ * single-precision squares convert to float, square in float and widen back.
+ * -------------------------------------------------------------------------- */
+static double weno_sq(double d, int single_squares)
+{
+    if (single_squares) {
+        float f = (float)d;
+        float f2 = f * f;
+        return (double)f2;
+    }
+    return d * d;
+}
+
+static double weno_flux(double v1, double v2, double v3, double v4, double v5, int ss)
+{
+    const double phi1 = ((1.0 / 3.0) * v1 - (7.0 / 6.0) * v2) + (11.0 / 6.0) * v3;
+    const double phi2 = (-(1.0 / 6.0) * v2 + (5.0 / 6.0) * v3) + (1.0 / 3.0) * v4;
+    const double phi3 = ((1.0 / 3.0) * v3 + (5.0 / 6.0) * v4) - (1.0 / 6.0) * v5;
+    const double s1 = (13.0 / 12.0) * weno_sq((v1 - 2.0 * v2) + v3, ss) +
+                      0.25 * weno_sq((v1 - 4.0 * v2) + 3.0 * v3, ss);
+    const double s2 = (13.0 / 12.0) * weno_sq((v2 - 2.0 * v3) + v4, ss) +
+                      0.25 * weno_sq(v2 - v4, ss);
+    const double s3 = (13.0 / 12.0) * weno_sq((v3 - 2.0 * v4) + v5, ss) +
+                      0.25 * weno_sq((3.0 * v3 - 4.0 * v4) + v5, ss);
+    const double a1 = 0.1 / weno_sq(s1 + 1e-06, ss);
+    const double a2 = 0.6 / weno_sq(s2 + 1e-06, ss);
+    const double a3 = 0.3 / weno_sq(s3 + 1e-06, ss);
+    const double denom = 1.0 / ((a1 + a2) + a3);
+    const double w1 = a1 * denom, w2 = a2 * denom, w3 = a3 * denom;
+    return (phi1 * w1 + phi2 * w2) + phi3 * w3;
+}
+
+void cso_weno_f64(const double* in, const double* u, const double* v, double* out,
+                  long nx, long ny, double coeDx, double coeDy, int single_squares,
+                  int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long y = 0; y < ny; y++) {
+        const double* row = in + y * nx;
+        for (long x = 0; x < nx; x++) {
+            double dx[7], dy[7];  /* d[k] = difference ending at offset k - 3, k = 1..6 */
+            for (int k = 1; k < 7; k++) {
+                const long xa = ((x + k - 3) % nx + nx) % nx, xb = ((x + k - 4) % nx + nx) % nx;
+                const long ya = ((y + k - 3) % ny + ny) % ny, yb = ((y + k - 4) % ny + ny) % ny;
+                dx[k] = (row[xa] - row[xb]) * coeDx;
+                dy[k] = (in[ya * nx + x] - in[yb * nx + x]) * coeDy;
+            }
+            const long i = y * nx + x;
+            double fx, fy;
+            if (u[i] > 0.0) fx = weno_flux(dx[1], dx[2], dx[3], dx[4], dx[5], single_squares);
+            else fx = weno_flux(dx[6], dx[5], dx[4], dx[3], dx[2], single_squares);
+            if (v[i] > 0.0) fy = weno_flux(dy[1], dy[2], dy[3], dy[4], dy[5], single_squares);
+            else fy = weno_flux(dy[6], dy[5], dy[4], dy[3], dy[2], single_squares);
+            out[i] = u[i] * fx + v[i] * fy;
+        }
+    }
+}
+
+/* --------------------------------------------------------------------------
  * Cahn-Hilliard ADI.
  *
  * Restates cuPentSpeedUp/serialCahnADITiming/serialCahnADI.c:

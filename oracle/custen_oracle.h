@@ -52,6 +52,17 @@ void cso_apply_fun_f64(int fun_id, int dir, const double* in, double* out,
                        long nx, long ny, const double* coe, int numCoe,
                        int H, int L, int V, int T, int periodic, int nthreads);
 
+/* ---- WENO5 upwind advection (xy periodic) ---------------------------------- */
+
+/* out = u * Fx + v * Fy with Fx, Fy the fifth-order WENO approximations of
+ * d(in)/dx, d(in)/dy, upwinded on the sign of u (resp. v), periodic in x and
+ * y.  coeDx = 1/dx, coeDy = 1/dy.  single_squares != 0 squares the smoothness
+ * terms in single precision (the reference evaluates them with powf on
+ * doubles); 0 squares them in double. */
+void cso_weno_f64(const double* in, const double* u, const double* v, double* out,
+                  long nx, long ny, double coeDx, double coeDy, int single_squares,
+                  int nthreads);
+
 /* ---- Cahn-Hilliard ADI ----------------------------------------------------- */
 
 typedef struct {

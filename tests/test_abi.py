@@ -51,6 +51,22 @@ def test_create_validation(args, msg):
     assert not h.plan
 
 
+@pytest.mark.parametrize("args,msg", [
+    # (dx, dy, u, v, out, in)
+    ((0.1, 0.1, None, 5, 2, 1), "null velocity"),
+    ((0.1, 0.1, 2, 5, 2, 1), "alias the output"),
+    ((0.1, 0.1, 4, 5, 1, 1), "must not alias"),
+    ((0.0, 0.1, 4, 5, 2, 1), "dx and dy"),
+    ((0.1, float("inf"), 4, 5, 2, 1), "dx and dy"),
+])
+def test_weno_create_validation(args, msg):
+    h = api.cuSten_t()
+    with pytest.raises(api.CustenError) as ei:
+        api.cuStenCreate2DXYWENOADVp(h, 0, 1, 64, 64, 32, 32, *args)
+    assert msg in str(ei.value)
+    assert not h.plan
+
+
 def test_uncreated_handle_is_rejected():
     h = api.cuSten_t()
     with pytest.raises(api.CustenError):
