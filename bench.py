@@ -10,8 +10,9 @@ HIP events on the stream the kernel is launched on), the CPU oracle baseline on
 a bounded sample of the same workload, and under "extra" the secondary
 figures: other stencils of the 8192^2 configuration (np, 9-point Laplacian,
 point function, FP32 storage with its error against the FP64 oracle), the
-1024^2 x-periodic configuration, batched uneven grids against the per-grid
-loop, and the served Cahn-Hilliard stream against the plain per-request loop.
+1024^2 x-periodic configuration, WENO5 advection at 8192^2, batched uneven
+grids against the per-grid loop, batched pentadiagonal factor and solve, and
+the served Cahn-Hilliard stream against the plain per-request loop.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --workload stream    # 65536 x 32768 host-resident tiled streaming
@@ -261,6 +262,66 @@ def bench_weno_8192(torch, api, stream, timer, din, dout, steps, extra):
     del du, dv, sphi, sout
 
 
+def bench_pent(args, torch, api, extra):
+    """Batched pentadiagonal factor + solve, 65536 interleaved systems of 1024
+    unknowns with their own diagonals.  Algorithmic bytes per unknown: factor
+    72 (read 5 diagonals, write 4), solve 72 (forward: read ds, dl, d, b and
+    write b; backward: read du, dw, b and write b)."""
+    import oracle
+
+    m, batch = 1024, 65536
+    n = m * batch
+    stream = torch.cuda.current_stream()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+
+    def rnd(a, b):
+        return torch.rand(m, batch, dtype=torch.float64, device="cuda", generator=gen) * (b - a) + a
+
+    pristine = [rnd(-1, 1), rnd(-1, 1), rnd(6, 9), rnd(-1, 1), rnd(-1, 1)]
+    rhs = rnd(-1, 1)
+    work = [torch.empty_like(a) for a in pristine]
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 5
+    fac_ms = 0.0
+    for r in range(reps + 1):
+        for w, p in zip(work, pristine):
+            w.copy_(p)
+        e0.record(stream)
+        api.custenPentFactorBatch(*work, m, batch, stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if r > 0:
+            fac_ms += e0.elapsed_time(e1) / reps
+    b = rhs.clone()
+    sol = lambda: api.custenPentSolveBatch(*work, b, m, batch,  # noqa: E731
+                                           stream=stream.cuda_stream)
+    sol_ms = Timer(torch, stream).run(sol, reps, 1)
+    # parity on the first 256 systems: factor again from pristine, one solve
+    for w, p in zip(work, pristine):
+        w.copy_(p)
+    b.copy_(rhs)
+    api.custenPentFactorBatch(*work, m, batch, stream=stream.cuda_stream)
+    api.custenPentSolveBatch(*work, b, m, batch, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    k = 256
+    host = [p[:, :k].cpu().numpy().copy() for p in pristine]
+    fac = oracle.pent_factor_batch(*host, nthreads=min(16, os.cpu_count() or 1))
+    ref = oracle.pent_solve_batch(*fac, rhs[:, :k].cpu().numpy().copy())
+    got = b[:, :k].cpu().numpy()
+    extra["pent_batch"] = {
+        "m": m, "systems": batch,
+        "factor_ms": round(fac_ms, 3), "factor_GB/s": round(72.0 * n / (fac_ms * 1e-3) / 1e9, 1),
+        "solve_ms": round(sol_ms, 3), "solve_GB/s": round(72.0 * n / (sol_ms * 1e-3) / 1e9, 1),
+        "systems_checked_vs_oracle": k,
+        "mismatching_unknowns": int(np.count_nonzero(got.view(np.uint64) != ref.view(np.uint64))),
+    }
+    log(f"pent batch: {extra['pent_batch']}")
+    del pristine, work, rhs, b
+    torch.cuda.empty_cache()
+
+
 def bench_x_1024(args, torch, api, extra):
     """1024^2 x-periodic 5-point d2/dx2 (cache-resident: effective GB/s)."""
     from custen_amd import synth
@@ -502,6 +563,7 @@ def main():
     if not args.no_extra:
         bench_x_1024(args, torch, api, extra)
         bench_batch(args, torch, api, extra)
+        bench_pent(args, torch, api, extra)
         bench_ch(args, torch, api, extra)
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):

@@ -137,6 +137,8 @@ for _pre in ("cuSten", "custen"):
     _SIGS[f"{_pre}Destroy2DXYWENOADVp"] = (None, [_H])
 _SIGS.update({
     "custenSetWenoSquares": (None, [_H, c_int]),
+    "custenPentFactorBatch": (c_int, [_P, _P, _P, _P, _P, c_int, c_int, c_void_p]),
+    "custenPentSolveBatch": (c_int, [_P, _P, _P, _P, _P, _P, c_int, c_int, c_void_p]),
     "cuSenCompute2DXpFun": (None, [_H, c_bool]),
     "checkError": (None, [c_char_p]),
     "custenGetLastError": (c_int, []),
@@ -319,6 +321,26 @@ for _pre in ("cuSten", "custen"):
     globals()[f"{_pre}Compute2DXYWENOADVp"] = _make_compute(f"{_pre}Compute2DXYWENOADVp")
     globals()[f"{_pre}Swap2DXYWENOADVp"] = _make_swap(f"{_pre}Swap2DXYWENOADVp")
     globals()[f"{_pre}Destroy2DXYWENOADVp"] = _make_destroy(f"{_pre}Destroy2DXYWENOADVp")
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+def custenPentFactorBatch(ds, dl, d, du, dw, m: int, batchCount: int, stream=None) -> None:
+    """In-place factorisation of interleaved pentadiagonal systems (device arrays)."""
+    lib().custenPentFactorBatch(_addr(ds), _addr(dl), _addr(d), _addr(du), _addr(dw), m,
+                                batchCount, _stream(stream))
+    check("custenPentFactorBatch")
+
+
+def custenPentSolveBatch(ds, dl, d, du, dw, b, m: int, batchCount: int, stream=None) -> None:
+    """Solve interleaved pentadiagonal systems in place of b (device arrays)."""
+    lib().custenPentSolveBatch(_addr(ds), _addr(dl), _addr(d), _addr(du), _addr(dw), _addr(b),
+                               m, batchCount, _stream(stream))
+    check("custenPentSolveBatch")
 
 
 def custenSetWenoSquares(h: cuSten_t, singlePrecision: bool) -> None:

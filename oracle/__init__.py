@@ -64,6 +64,8 @@ def lib():
         L.cso_pent_solve.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _i]
         L.cso_fill_uniform.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_double,
                                        ctypes.c_double]
+        L.cso_pent_factor_batch.argtypes = [_dp, _dp, _dp, _dp, _dp, _i, _l, _i]
+        L.cso_pent_solve_batch.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _i, _l, _i]
         L.cso_weno_f64.argtypes = [_dp, _dp, _dp, _dp, _l, _l, ctypes.c_double, ctypes.c_double,
                                    _i, _i]
         _lib = L
@@ -106,6 +108,25 @@ def apply_fun(fun_id, direction, inp, coe, H, L, V, T, periodic, out=None, nthre
     lib().cso_apply_fun_f64(fun_id, direction, _d(inp), _d(out), nx, ny, _d(coe), coe.size, H, L,
                             V, T, int(periodic), nthreads)
     return out
+
+
+def pent_factor_batch(ds, dl, d, du, dw, nthreads=1):
+    """Factorisation of interleaved (m, batch) diagonal arrays
+    (cuPentBatch.cu:35-113); returns the factored copies."""
+    arrs = [np.array(a, dtype=np.float64, copy=True, order="C") for a in (ds, dl, d, du, dw)]
+    m, batch = arrs[0].shape
+    lib().cso_pent_factor_batch(*[_d(a) for a in arrs], m, batch, nthreads)
+    return arrs
+
+
+def pent_solve_batch(ds, dl, d, du, dw, b, nthreads=1):
+    """Solve interleaved (m, batch) systems with factored diagonals
+    (cuPentBatch.cu:119-198); returns the solution array."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ds, dl, d, du, dw)]
+    x = np.array(b, dtype=np.float64, copy=True, order="C")
+    m, batch = x.shape
+    lib().cso_pent_solve_batch(*[_d(a) for a in arrs], _d(x), m, batch, nthreads)
+    return x
 
 
 def weno(inp, u, v, dx, dy, single_squares=True, out=None, nthreads=1):

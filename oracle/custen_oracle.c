@@ -276,55 +276,90 @@ void cso_weno_f64(const double* in, const double* u, const double* v, double* ou
  * (cuPentCahnADI/src/cuPentCahnADI.cu:537-540).
  * -------------------------------------------------------------------------- */
 
-void cso_pent_factor(double* ds, double* dl, double* d, double* du, double* dw, int m)
+static void pent_factor_strided(double* ds, double* dl, double* d, double* du, double* dw,
+                                int m, long s)
 {
-    /* serialCahnADI.c:91-155 */
-    int r = 0;
+    /* serialCahnADI.c:91-155; cuPentBatch.cu:35-113 is the same recurrence on
+     * interleaved systems (element i of a system at i * s) */
+    long r = 0;
     du[r] = du[r] / d[r];
     dw[r] = dw[r] / d[r];
-    int rp = r;
-    r += 1;
+    long rp = r;
+    r += s;
     d[r] = d[r] - dl[r] * du[rp];
     du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
     dw[r] = dw[r] / d[r];
     for (int i = 2; i < m - 2; i++) {
-        int rpp = r - 1;
+        long rpp = r - s;
         rp = r;
-        r += 1;
+        r += s;
         dl[r] = dl[r] - ds[r] * du[rpp];
         d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
         dw[r] = dw[r] / d[r];
         du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
     }
     {
-        int rpp = r - 1;
+        long rpp = r - s;
         rp = r;
-        r += 1;
+        r += s;
         dl[r] = dl[r] - ds[r] * du[rpp];
         d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
         du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
     }
     {
-        int rpp = r - 1;
+        long rpp = r - s;
         rp = r;
-        r += 1;
+        r += s;
         dl[r] = dl[r] - ds[r] * du[rpp];
         d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
     }
+}
+
+void cso_pent_factor(double* ds, double* dl, double* d, double* du, double* dw, int m)
+{
+    pent_factor_strided(ds, dl, d, du, dw, m, 1);
+}
+
+/* Diagonals with element stride ds_s (0 for diagonals shared by every line),
+ * right-hand side with stride s. */
+static void pent_solve_strided2(const double* ds, const double* dl, const double* d,
+                                const double* du, const double* dw, long q, double* b, int m,
+                                long s)
+{
+    /* serialCahnADI.c:161-241; cuPentBatch.cu:119-198 */
+    b[0] = b[0] / d[0];
+    b[s] = (b[s] - dl[q] * b[0]) / d[q];
+    for (int i = 2; i < m; i++)
+        b[i * s] = (b[i * s] - ds[i * q] * b[(i - 2) * s] - dl[i * q] * b[(i - 1) * s]) / d[i * q];
+    b[(m - 2) * s] = b[(m - 2) * s] - du[(m - 2) * q] * b[(m - 1) * s];
+    for (int i = m - 3; i >= 0; i--)
+        b[i * s] = b[i * s] - du[i * q] * b[(i + 1) * s] - dw[i * q] * b[(i + 2) * s];
 }
 
 static void pent_solve_strided(const double* ds, const double* dl, const double* d,
                                const double* du, const double* dw, double* b, int m,
                                long s)
 {
-    /* serialCahnADI.c:161-241 */
-    b[0] = b[0] / d[0];
-    b[s] = (b[s] - dl[1] * b[0]) / d[1];
-    for (int i = 2; i < m; i++)
-        b[i * s] = (b[i * s] - ds[i] * b[(i - 2) * s] - dl[i] * b[(i - 1) * s]) / d[i];
-    b[(m - 2) * s] = b[(m - 2) * s] - du[m - 2] * b[(m - 1) * s];
-    for (int i = m - 3; i >= 0; i--)
-        b[i * s] = b[i * s] - du[i] * b[(i + 1) * s] - dw[i] * b[(i + 2) * s];
+    pent_solve_strided2(ds, dl, d, du, dw, 1, b, m, s);
+}
+
+void cso_pent_factor_batch(double* ds, double* dl, double* d, double* du, double* dw, int m,
+                           long batch, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long k = 0; k < batch; k++)
+        pent_factor_strided(ds + k, dl + k, d + k, du + k, dw + k, m, batch);
+}
+
+void cso_pent_solve_batch(const double* ds, const double* dl, const double* d,
+                          const double* du, const double* dw, double* b, int m, long batch,
+                          int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long k = 0; k < batch; k++)
+        pent_solve_strided2(ds + k, dl + k, d + k, du + k, dw + k, batch, b + k, m, batch);
 }
 
 void cso_pent_solve(const double* ds, const double* dl, const double* d,

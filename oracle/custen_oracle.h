@@ -90,6 +90,13 @@ void cso_ch_cyclic_solve(const cso_ch_t* ch, double* x, long stride);
 void cso_pent_factor(double* ds, double* dl, double* d, double* du, double* dw, int m);
 void cso_pent_solve(const double* ds, const double* dl, const double* d,
                     const double* du, const double* dw, double* b, int m);
+/* The same on `batch` interleaved systems (element i of system k at
+ * i * batch + k), as cuPentBatch's pentFactorBatch / pentSolveBatch. */
+void cso_pent_factor_batch(double* ds, double* dl, double* d, double* du, double* dw, int m,
+                           long batch, int nthreads);
+void cso_pent_solve_batch(const double* ds, const double* dl, const double* d,
+                          const double* du, const double* dw, double* b, int m, long batch,
+                          int nthreads);
 
 /* ---- synthetic inputs ------------------------------------------------------- */
 

@@ -67,6 +67,16 @@ def test_weno_create_validation(args, msg):
     assert not h.plan
 
 
+@pytest.mark.parametrize("m,batch,msg", [(3, 4, "m must be >= 4"), (8, 0, "batchCount")])
+def test_pent_validation_without_gpu(m, batch, msg):
+    with pytest.raises(api.CustenError) as ei:
+        api.custenPentFactorBatch(1, 2, 3, 4, 5, m, batch)
+    assert msg in str(ei.value)
+    with pytest.raises(api.CustenError) as ei:
+        api.custenPentSolveBatch(1, 2, 3, 4, 5, 6, m, batch)
+    assert msg in str(ei.value)
+
+
 def test_uncreated_handle_is_rejected():
     h = api.cuSten_t()
     with pytest.raises(api.CustenError):

@@ -291,3 +291,32 @@ def test_splitmix_implementations_agree():
     c = np.empty(4096)
     api.custenFillUniform(c, 4096, 123, -0.1, 0.1)
     assert bits_equal(a, c)
+
+
+def _pent_systems(m, batch, seed):
+    """Interleaved (m, batch) diagonals of diagonally dominant systems."""
+    ds, dl, du, dw = (synth.field(batch, m, seed=seed + k) for k in range(4))
+    d = synth.field(batch, m, seed=seed + 4, a=6.0, b=9.0)
+    ds[:2] = 0.0
+    dl[:1] = 0.0
+    du[-1:] = 0.0
+    dw[-2:] = 0.0
+    b = synth.field(batch, m, seed=seed + 5)
+    return ds, dl, d, du, dw, b
+
+
+@pytest.mark.parametrize("m,batch", [(4, 1), (5, 3), (12, 7), (33, 2)])
+def test_pent_batch_matches_dense_and_single(m, batch):
+    """Batched interleaved factor + solve: each system's solution matches a dense
+    LU solve to rounding level and the single-system oracle bit for bit."""
+    ds, dl, d, du, dw, b = _pent_systems(m, batch, 300 + m)
+    fac = oracle.pent_factor_batch(ds, dl, d, du, dw)
+    x = oracle.pent_solve_batch(*fac, b)
+    for k in range(batch):
+        A = (np.diag(d[:, k]) + np.diag(dl[1:, k], -1) + np.diag(ds[2:, k], -2) +
+             np.diag(du[:-1, k], 1) + np.diag(dw[:-2, k], 2))
+        ref = np.linalg.solve(A, b[:, k])
+        assert np.abs(x[:, k] - ref).max() < 1e-13 * np.abs(ref).max()
+        single = oracle.pent_factor_solve(ds[:, k], dl[:, k], d[:, k], du[:, k], dw[:, k],
+                                          b[:, k])
+        assert bits_equal(x[:, k], single)
