@@ -176,6 +176,12 @@ _SIGS.update({
                                     ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     "custenCHServerResult": (c_int, [c_void_p, c_int, _P]),
     "custenCHFootprint": (c_size_t, [c_int]),
+    "custenCHStepsDone": (c_int, [c_void_p]),
+    "custenCHSnapshot": (c_int, [c_void_p, c_char_p]),
+    "custenCHCheckpoint": (c_int, [c_void_p, c_char_p]),
+    "custenCHRestore": (c_void_p, [c_int, c_char_p]),
+    "custenCHServerSubmitResume": (c_int, [c_void_p, ctypes.POINTER(custenCHParams), c_int, _P,
+                                           _P]),
     "custenCHServerDestroy": (None, [c_void_p]),
 })
 
@@ -417,6 +423,35 @@ class CHSolver:
             check("custenCHCreate")
             raise CustenError(2, "custenCHCreate failed")
 
+    @classmethod
+    def restore(cls, path: str, device: int = 0) -> "CHSolver":
+        """A solver continuing from a checkpoint file (custenCHRestore)."""
+        h = lib().custenCHRestore(device, os.fsencode(path))
+        if not h:
+            check("custenCHRestore")
+            raise CustenError(1, "custenCHRestore failed")
+        self = cls.__new__(cls)
+        self.h = h
+        self._p = None
+        # the grid size is the checkpoint's (header bytes 12..15, u32 n)
+        with open(path, "rb") as f:
+            head = f.read(16)
+        self.n = int.from_bytes(head[12:16], "little")
+        return self
+
+    @property
+    def steps_done(self) -> int:
+        return lib().custenCHStepsDone(self.h)
+
+    def checkpoint(self, path: str) -> None:
+        if lib().custenCHCheckpoint(self.h, os.fsencode(path)) != 0:
+            check("custenCHCheckpoint")
+
+    def snapshot(self, path: str) -> None:
+        """Write the current field c as a .npy file (custenCHSnapshot)."""
+        if lib().custenCHSnapshot(self.h, os.fsencode(path)) != 0:
+            check("custenCHSnapshot")
+
     def run(self, steps: int, mode: int = 0, stream: int = 0) -> None:
         rc = lib().custenCHRun(self.h, steps, mode, stream)
         if rc != 0:
@@ -453,8 +488,13 @@ class CHServer:
             check("custenCHServerCreate")
         self._n = {}
 
-    def submit(self, params: custenCHParams, steps: int, c0) -> int:
-        rid = lib().custenCHServerSubmit(self.h, ctypes.byref(params), steps, _addr(c0))
+    def submit(self, params: custenCHParams, steps: int, c0, c_old=None) -> int:
+        """Queue a request; with c_old, resume from the time levels (c0, c_old)."""
+        if c_old is None:
+            rid = lib().custenCHServerSubmit(self.h, ctypes.byref(params), steps, _addr(c0))
+        else:
+            rid = lib().custenCHServerSubmitResume(self.h, ctypes.byref(params), steps,
+                                                   _addr(c0), _addr(c_old))
         if rid < 0:
             check("custenCHServerSubmit")
         self._n[rid] = params.n

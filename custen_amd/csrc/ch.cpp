@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -230,6 +232,7 @@ struct CHRequest {
     int admitStep = 0;
     CHBuffers buf;
     std::vector<double> c0;
+    std::vector<double> cOld0;  // empty: cOld starts equal to c0
     std::vector<double> result;
     int state = 0;  // 0 queued, 1 resident, 2 finished
     Clock::time_point tSubmit, tAdmit, tFinish;
@@ -247,15 +250,16 @@ public:
 
     int step() const { return step_; }
 
-    hipError_t admit(CHRequest* r, const double* c0, hipStream_t st)
+    // cOld0 == nullptr: cOld starts equal to c (cuPentCahnADI.cu:295-305);
+    // otherwise the pair (c0, cOld0) resumes a simulation exactly.
+    hipError_t admit(CHRequest* r, const double* c0, const double* cOld0, hipStream_t st)
     {
         const size_t bytes = (size_t)r->co.n * r->co.n * sizeof(double);
         hipError_t e;
         if ((e = hipMemcpyAsync(r->buf.f[0], c0, bytes, hipMemcpyDefault, st)) != hipSuccess) return e;
-        // cOld starts equal to c (cuPentCahnADI.cu:295-305)
-        if ((e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st)) !=
-            hipSuccess)
-            return e;
+        if (cOld0) e = hipMemcpyAsync(r->buf.f[1], cOld0, bytes, hipMemcpyDefault, st);
+        else e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
         if ((e = hipMemcpyAsync(r->buf.fac, r->co.fac.data(), r->co.fac.size() * sizeof(double),
                                 hipMemcpyHostToDevice, st)) != hipSuccess)
             return e;
@@ -450,45 +454,192 @@ hipError_t preparePlain(custenCH* ch)
 
 }  // namespace
 
-extern "C" custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0)
+namespace {
+
+custenCH* createSolver(const char* fn, int deviceNum, const custenCHParams* params,
+                       const double* c0, const double* cOld0, int stepsDone)
 {
     if (params == nullptr || c0 == nullptr) {
-        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCreate: null argument");
+        setError(CUSTEN_ERROR_INVALID_VALUE, std::string(fn) + ": null argument");
         return nullptr;
     }
     std::unique_ptr<custenCH> ch(new custenCH());
     std::string why;
     if (!chSetup(*params, ch->req.co, why)) {
-        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCreate: " + why);
+        setError(CUSTEN_ERROR_INVALID_VALUE, std::string(fn) + ": " + why);
         return nullptr;
     }
     hipError_t e = hipSetDevice(deviceNum);
     if (e != hipSuccess) {
-        setHipError(e, "custenCHCreate");
+        setHipError(e, fn);
         return nullptr;
     }
     ch->device = deviceNum;
     if ((e = hipStreamCreate(&ch->stream)) != hipSuccess) {
-        setHipError(e, "custenCHCreate");
+        setHipError(e, fn);
         return nullptr;
     }
     ch->engine.reset(new CHEngine(deviceNum));
     ch->pool.reset(new BufferPool(0));
     ch->req.params = *params;
+    ch->req.done = stepsDone;
     if ((e = ch->pool->get(params->n, ch->req.buf)) != hipSuccess) {
-        setHipError(e, "custenCHCreate");
+        setHipError(e, fn);
         (void)hipStreamDestroy(ch->stream);
         return nullptr;
     }
-    if ((e = ch->engine->admit(&ch->req, c0, ch->stream)) != hipSuccess ||
+    if ((e = ch->engine->admit(&ch->req, c0, cOld0, ch->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ch->stream)) != hipSuccess) {
-        setHipError(e, "custenCHCreate");
+        setHipError(e, fn);
         ch->pool->put(ch->req.buf);
         (void)hipStreamDestroy(ch->stream);
         return nullptr;
     }
     ch->req.state = 1;
     return ch.release();
+}
+
+// Checkpoint file: "CUSTENCH", u32 version, u32 n, f64 lx, D, gamma, dtOverDx,
+// i64 steps done, then c and cOld as n*n little-endian doubles each.
+constexpr char kCkptMagic[8] = {'C', 'U', 'S', 'T', 'E', 'N', 'C', 'H'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr size_t kCkptHeader = 8 + 4 + 4 + 4 * 8 + 8;
+
+bool writeFileAtomic(const std::string& path, const std::vector<std::pair<const void*, size_t>>& parts,
+                     std::string& why)
+{
+    const std::string tmp = path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) {
+        why = "cannot open " + tmp + " for writing";
+        return false;
+    }
+    bool ok = true;
+    for (const auto& p : parts)
+        ok = ok && std::fwrite(p.first, 1, p.second, f) == p.second;
+    ok = (std::fclose(f) == 0) && ok;
+    if (ok && std::rename(tmp.c_str(), path.c_str()) != 0) ok = false;
+    if (!ok) {
+        std::remove(tmp.c_str());
+        why = "write to " + path + " failed";
+    }
+    return ok;
+}
+
+}  // namespace
+
+extern "C" custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0)
+{
+    return createSolver("custenCHCreate", deviceNum, params, c0, nullptr, 0);
+}
+
+extern "C" int custenCHStepsDone(custenCH* ch)
+{
+    if (ch == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHStepsDone: null handle");
+        return -1;
+    }
+    return ch->req.done;
+}
+
+extern "C" int custenCHCheckpoint(custenCH* ch, const char* path)
+{
+    if (ch == nullptr || path == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCheckpoint: null argument");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    const int n = ch->req.co.n;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> c(nn), cOld(nn);
+    int rc = custenCHGetField(ch, c.data(), cOld.data());
+    if (rc != CUSTEN_SUCCESS) return rc;
+    const custenCHParams& pr = ch->req.params;
+    const uint32_t ver = kCkptVersion, un = (uint32_t)n;
+    const double prm[4] = {pr.lx, pr.D, pr.gamma, pr.dtOverDx};
+    const int64_t done = ch->req.done;
+    std::string why;
+    if (!writeFileAtomic(path, {{kCkptMagic, 8}, {&ver, 4}, {&un, 4}, {prm, sizeof(prm)},
+                                {&done, 8}, {c.data(), nn * 8}, {cOld.data(), nn * 8}},
+                         why)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHCheckpoint: " + why);
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" custenCH* custenCHRestore(int deviceNum, const char* path)
+{
+    const char* fn = "custenCHRestore";
+    if (path == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, std::string(fn) + ": null path");
+        return nullptr;
+    }
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, std::string(fn) + ": cannot open " + path);
+        return nullptr;
+    }
+    char magic[8];
+    uint32_t ver = 0, un = 0;
+    double prm[4];
+    int64_t done = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 &&
+              std::fread(&ver, 4, 1, f) == 1 && ver == kCkptVersion &&
+              std::fread(&un, 4, 1, f) == 1 && un >= 8 && un <= (uint32_t)kCHMaxN &&
+              std::fread(prm, sizeof(prm), 1, f) == 1 && std::fread(&done, 8, 1, f) == 1 &&
+              done >= 0 && done <= 0x7fffffff;
+    std::vector<double> c, cOld;
+    if (ok) {
+        const size_t nn = (size_t)un * un;
+        c.resize(nn);
+        cOld.resize(nn);
+        ok = std::fread(c.data(), 8, nn, f) == nn && std::fread(cOld.data(), 8, nn, f) == nn &&
+             std::fgetc(f) == EOF;
+    }
+    std::fclose(f);
+    if (!ok) {
+        setError(CUSTEN_ERROR_INVALID_VALUE,
+                 std::string(fn) + ": " + path + " is not a complete checkpoint of this format");
+        return nullptr;
+    }
+    custenCHParams pr{};
+    pr.n = (int)un;
+    pr.lx = prm[0];
+    pr.D = prm[1];
+    pr.gamma = prm[2];
+    pr.dtOverDx = prm[3];
+    return createSolver(fn, deviceNum, &pr, c.data(), cOld.data(), (int)done);
+}
+
+extern "C" int custenCHSnapshot(custenCH* ch, const char* path)
+{
+    if (ch == nullptr || path == nullptr) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHSnapshot: null argument");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    const int n = ch->req.co.n;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> c(nn);
+    int rc = custenCHGetField(ch, c.data(), nullptr);
+    if (rc != CUSTEN_SUCCESS) return rc;
+    // NumPy .npy version 1.0: magic, version, header length, header dict
+    // padded with spaces to a 64-byte boundary and ended by a newline
+    std::string dict = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + std::to_string(n) +
+                       ", " + std::to_string(n) + "), }";
+    const size_t pre = 10;
+    size_t total = pre + dict.size() + 1;
+    dict.append((64 - total % 64) % 64, ' ');
+    dict.push_back('\n');
+    const uint16_t hlen = (uint16_t)dict.size();
+    const unsigned char head[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    std::string why;
+    if (!writeFileAtomic(path, {{head, 8}, {&hlen, 2}, {dict.data(), dict.size()},
+                                {c.data(), nn * 8}},
+                         why)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHSnapshot: " + why);
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    return CUSTEN_SUCCESS;
 }
 
 extern "C" int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream)
@@ -632,8 +783,8 @@ extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetB
     return s.release();
 }
 
-extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
-                                    const double* c0)
+extern "C" int custenCHServerSubmitResume(custenCHServer* s, const custenCHParams* params,
+                                          int steps, const double* c0, const double* cOld0)
 {
     if (s == nullptr || params == nullptr || c0 == nullptr || steps < 1) {
         setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerSubmit: bad argument");
@@ -655,10 +806,17 @@ extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* par
     r->steps = steps;
     const size_t nn = (size_t)params->n * params->n;
     r->c0.assign(c0, c0 + nn);
+    if (cOld0) r->cOld0.assign(cOld0, cOld0 + nn);
     r->tSubmit = Clock::now();
     s->queue.push_back(r.get());
     s->all.push_back(std::move(r));
     return s->all.back()->id;
+}
+
+extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
+                                    const double* c0)
+{
+    return custenCHServerSubmitResume(s, params, steps, c0, nullptr);
 }
 
 namespace {
@@ -679,7 +837,8 @@ int serverIteration(custenCHServer* s)
             }
             break;  // wait for memory to free up
         }
-        if ((e = s->engine->admit(r, r->c0.data(), s->stream)) != hipSuccess) {
+        if ((e = s->engine->admit(r, r->c0.data(), r->cOld0.empty() ? nullptr : r->cOld0.data(),
+                                  s->stream)) != hipSuccess) {
             setHipError(e, "custenCHServerStep (admit)");
             return -1;
         }
@@ -716,6 +875,7 @@ int serverIteration(custenCHServer* s)
             r->buf = CHBuffers{};
             s->used -= footprintBytes(r->co.n);
             std::vector<double>().swap(r->c0);
+            std::vector<double>().swap(r->cOld0);
         }
     }
     return (int)(s->queue.size() + s->engine->active().size());

@@ -40,6 +40,23 @@ int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream);
 int custenCHGetField(custenCH* ch, double* c, double* cOld);
 void custenCHDestroy(custenCH* ch);
 
+/* Steps taken so far (including those recorded in a restored checkpoint). */
+int custenCHStepsDone(custenCH* ch);
+
+/* Snapshot output, the counterpart of Print_Out (cuPentCahnADI.cu:103-140,
+ * which writes c to an HDF5 file per output time): the current field c as a
+ * NumPy .npy file (float64, shape (n, n), C order).  Written to a temporary
+ * name and renamed, so a reader never sees a partial file. */
+int custenCHSnapshot(custenCH* ch, const char* path);
+
+/* Checkpoint / restart (no counterpart in the reference): the parameters, the
+ * step count and both time levels c and cOld, so that a restored solver
+ * continues bit for bit as the original would have.  Format: "CUSTENCH",
+ * u32 version (1), u32 n, f64 lx, D, gamma, dtOverDx, i64 steps done, then c
+ * and cOld as n*n little-endian doubles each. */
+int custenCHCheckpoint(custenCH* ch, const char* path);
+custenCH* custenCHRestore(int deviceNum, const char* path);
+
 /* Request server with continuous batching.  Requests are admitted while their
  * resident footprint fits in `hbmBudgetBytes` and a slot is free; every engine
  * iteration advances all resident requests by one step with one batched
@@ -49,6 +66,9 @@ custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetBytes, int m
 /* Queue a request.  Returns its id (>= 0) or -1.  c0 is copied at submit. */
 int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
                          const double* c0);
+/* The same for a simulation resumed from a checkpoint: both time levels. */
+int custenCHServerSubmitResume(custenCHServer* s, const custenCHParams* params, int steps,
+                               const double* c, const double* cOld);
 /* Run engine iterations until the queue is empty and all requests finished
  * (maxIterations <= 0: no limit).  Returns the number of iterations. */
 int custenCHServerRunUntilIdle(custenCHServer* s, int maxIterations);

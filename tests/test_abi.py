@@ -115,3 +115,18 @@ def test_batch_validation_without_gpu():
     assert not L.custenBatchCreate(0, 1, nx, nx, ptr, ptr, w, 7, 3, 7, 3, 0)
     assert L.custenGetLastError() == 4  # shape not supported for batches
     api.clear_error()
+
+
+def test_ch_restore_rejects_bad_files(tmp_path):
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(b"NOTACKPT" + bytes(48))
+    with pytest.raises(api.CustenError, match="not a complete checkpoint"):
+        api.CHSolver.restore(str(bad))
+    short = tmp_path / "short.ckpt"
+    import struct
+    short.write_bytes(b"CUSTENCH" + struct.pack("<II4dq", 1, 16, 6.28, 1.0, 0.01, 0.1, 3) +
+                      bytes(16 * 16 * 8))  # cOld missing
+    with pytest.raises(api.CustenError, match="not a complete checkpoint"):
+        api.CHSolver.restore(str(short))
+    with pytest.raises(api.CustenError, match="cannot open"):
+        api.CHSolver.restore(str(tmp_path / "missing.ckpt"))
