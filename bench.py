@@ -16,6 +16,7 @@ the served Cahn-Hilliard stream against the plain per-request loop.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --workload stream    # 65536 x 32768 host-resident tiled streaming
+  python bench.py --workload weno|pent|batch|ch   # one secondary benchmark alone
 """
 from __future__ import annotations
 
@@ -96,14 +97,14 @@ def destroy(api, kind, h, f32=False):
         getattr(api, f"cuStenDestroy{_NAMES[kind]}")(h)
 
 
-def cpu_baseline(n: int, w: np.ndarray, seconds: float):
+def cpu_baseline(n: int, w: np.ndarray, seconds: float, threads: int = 0):
     """Oracle (fma chain, OpenMP over rows) on 8192 x 1024 slabs of the same
-    workload, repeated for about `seconds`."""
+    workload, repeated for about `seconds`; threads 0 = up to 16 host cores."""
     import oracle
     from custen_amd import synth
 
-    threads = min(16, os.cpu_count() or 1)
-    rows = 1024
+    threads = threads or min(16, os.cpu_count() or 1)
+    rows = 1024 if threads > 1 else 128
     slab = synth.uniform(n * rows, 2, -1.0, 1.0).reshape(rows, n)
     out = np.empty_like(slab)
     oracle.apply(slab, w, 5, 2, 5, 2, True, out=out, nthreads=threads)  # warm
@@ -143,9 +144,19 @@ def bench_stencil_8192(args, torch, api, extra):
     wb = torch.from_numpy(wbh).cuda()
 
     h = make_handle(api, stream, "xy_p_bih", n, dout, din, wb)
+    if args.tile:
+        api.custenSetTileConfig(h, *(int(v) for v in args.tile.split(",")))
     api.cuStenCompute2DXYp(h, api.DEVICE)  # autotunes the tile configuration once
     torch.cuda.synchronize()
-    ms = timer.run(compute_fn(api, "xy_p_bih", h), args.steps, args.warmup)
+    # untimed: ~100 ms of applies so that the GPU clocks have ramped up whether
+    # or not the autotuner ran, then the W warm-up steps and the K timed steps
+    fn = compute_fn(api, "xy_p_bih", h)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.1:
+        for _ in range(50):
+            fn()
+        torch.cuda.synchronize()
+    ms = timer.run(fn, args.steps, args.warmup)
     rpw, chunks, cpl = api.custenGetTileConfig(h)
     extra["headline_tile_config"] = {"rows_per_wave": rpw, "chunks": chunks,
                                      "columns_per_lane": cpl}
@@ -158,7 +169,26 @@ def bench_stencil_8192(args, torch, api, extra):
     extra["headline_mismatching_points"] = int(np.count_nonzero(
         got[inner].view(np.uint64) != ref[inner].view(np.uint64)))
     log(f"xy_p_bih 8192^2: {ms * 1e3:.1f} us/apply")
+    # measured copy ceiling: device-to-device copy of the same 512 MiB (also the
+    # calibration dispatch for the rocprof traffic counters)
+    tc = timer.run(lambda: dout.copy_(din), max(5, args.steps // 4), 2)
+    extra["copy_ceiling_GBps"] = round(2 * 8.0 * pts / (tc * 1e-3) / 1e9, 1)
+    if not args.no_extra:
+        bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref)
+    del din, dout
+    torch.cuda.empty_cache()
+    return ms
 
+
+def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
+    """The other stencils of the 8192^2 configuration on the headline's input."""
+    from custen_amd import workloads
+
+    n = din.shape[0]
+    pts = n * n
+    stream = torch.cuda.current_stream()
+    timer = Timer(torch, stream)
+    inner = (slice(2, 258), slice(2, n - 2))
     k2 = max(5, args.steps // 4)
     wl = torch.from_numpy(workloads.laplacian9_weights(n)).cuda()
     fptr = api.custenGetBuiltinFunction("xy_cube_minus", 0)
@@ -199,12 +229,6 @@ def bench_stencil_8192(args, torch, api, extra):
 
     bench_weno_8192(torch, api, stream, timer, din, dout, k2, extra)
 
-    # measured copy ceiling: device-to-device copy of the same 512 MiB
-    tc = timer.run(lambda: dout.copy_(din), k2, 2)
-    extra["copy_ceiling_GBps"] = round(2 * 8.0 * pts / (tc * 1e-3) / 1e9, 1)
-    del din, dout
-    torch.cuda.empty_cache()
-    return ms
 
 
 def bench_weno_8192(torch, api, stream, timer, din, dout, steps, extra):
@@ -478,7 +502,52 @@ def bench_ch(args, torch, api, extra):
     srv.close()
     out["served_saturated"] = {"solves_per_s": round(count / t_sat, 2),
                                "speedup_vs_plain": round((count / t_sat) / plain_rate, 2)}
+    out["cpu_oracle"] = ch_cpu_baseline(reqs, ics, steps)
     extra["served_cahn_hilliard"] = out
+
+
+def ch_cpu_baseline(reqs, ics, steps, sample_steps=2):
+    """The oracle's restatement of serialCahnADI.c on the host cores: every
+    request advanced `sample_steps` steps, single-threaded (as the reference's
+    serial program runs) and with one thread per request on up to 16 cores;
+    solves/s scaled to `steps` steps per request."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+
+    def advance(r):
+        n = reqs[r][1]
+        c = ics[r].ravel().copy()
+        co = c.copy()
+        oracle.CH(n).step(c, co, steps=sample_steps)
+
+    idx = list(range(len(reqs)))
+    t0 = time.perf_counter()
+    for r in idx:
+        advance(r)
+    t1 = time.perf_counter() - t0
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(advance, idx))
+    tp = time.perf_counter() - t0
+    scale = steps / sample_steps
+    return {"single_thread_solves_per_s": round(len(idx) / (t1 * scale), 4),
+            "threads_over_requests": threads,
+            "parallel_solves_per_s": round(len(idx) / (tp * scale), 4),
+            "sample": f"{len(idx)} requests x {sample_steps} steps, scaled to {steps} steps"}
+
+
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
 
 
 def bench_stream(args):
@@ -537,9 +606,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="stencil", choices=["stencil", "stream"])
+    ap.add_argument("--workload", default="stencil",
+                    choices=["stencil", "stream", "weno", "pent", "batch", "ch"],
+                    help="stencil: the headline line (default); the others run one "
+                         "secondary benchmark alone and print its figures")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--tile", default="",
+                    help="rows_per_wave,chunks,columns_per_lane for the headline apply "
+                         "(default: autotuned)")
     ap.add_argument("--ch-requests", type=int, default=24)
     ap.add_argument("--ch-steps", type=int, default=200)
     ap.add_argument("--ch-slots", type=int, default=64)
@@ -551,6 +626,29 @@ def main():
         sys.exit(2)
     if args.workload == "stream":
         bench_stream(args)
+        return
+    if args.workload in ("weno", "pent", "batch", "ch"):
+        import torch
+
+        from custen_amd import api
+
+        extra = {}
+        if args.workload == "weno":
+            n = 8192
+            host = np.empty((n, n))
+            api.custenFillUniform(host, n * n, 2, -1.0, 1.0)
+            din = torch.from_numpy(host).cuda()
+            dout = torch.empty_like(din)
+            stream = torch.cuda.current_stream()
+            bench_weno_8192(torch, api, stream, Timer(torch, stream), din, dout,
+                            max(5, args.steps // 4), extra)
+        elif args.workload == "pent":
+            bench_pent(args, torch, api, extra)
+        elif args.workload == "ch":
+            bench_ch(args, torch, api, extra)
+        else:
+            bench_batch(args, torch, api, extra)
+        print(json.dumps(extra), flush=True)
         return
 
     import torch
@@ -596,6 +694,9 @@ def main():
         "cpu_baseline": cpu_baseline(n, workloads.biharmonic_weights(n), args.cpu_seconds),
         "extra": extra,
     }
+    extra["cpu_host"] = host_cpu()
+    extra["cpu_baseline_single_thread"] = cpu_baseline(
+        n, workloads.biharmonic_weights(n), max(0.2, args.cpu_seconds / 3), threads=1)
     print(json.dumps(result), flush=True)
 
 

@@ -163,46 +163,93 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 // pentSolve (serialCahnADI.c:161-241) + solveEnd (:628-655); the column's
 // solveFull is applied by the next phase.  which = 0: Y, which = 1: X.
 // ---------------------------------------------------------------------------
-constexpr int PF = 16;
+constexpr int PF = 16;  // rows of the column in registers ahead of the recurrence
+constexpr int DB = 64;  // rows of the shared diagonals staged in LDS per block
+
+// Global-address-space views of descriptor pointers: loads through them are
+// global (not flat) loads, which the waitcnt logic tracks separately from LDS.
+using gdouble = __attribute__((address_space(1))) double;
 
 __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
                                                      int which)
 {
+    // Diagonals of the current and the next DB-row block (every column of a
+    // request shares one matrix): staged in LDS one block ahead, read as
+    // broadcasts.  Forward uses ds, dl, d; backward du, dw.
+    __shared__ double sd[2][3][DB];
     const int item = blockIdx.x;
     const int r = findRequest(descs, count, item, 2);
     const CHDesc& D = descs[r];
     const int n = D.n, m = n - 2;
-    const int col = (item - D.colBase) * kCHColBlock + (int)threadIdx.x;
-    if (col >= n) return;
-    double* __restrict__ Xc = (which ? D.X : D.Y) + col;
+    const int lane = (int)threadIdx.x;
+    const int col = (item - D.colBase) * kCHColBlock + lane;
+    const bool live = col < n;
+    gdouble* __restrict__ Xc = (gdouble*)((which ? D.X : D.Y) + (live ? col : 0));
     const int64_t s = n;
-    const double* __restrict__ ds = D.fac;
-    const double* __restrict__ dl = ds + m;
-    const double* __restrict__ dd = dl + m;
-    const double* __restrict__ du = dd + m;
-    const double* __restrict__ dw = du + m;
+    const gdouble* __restrict__ ds = (const gdouble*)D.fac;
+    const gdouble* __restrict__ dl = ds + m;
+    const gdouble* __restrict__ dd = dl + m;
+    const gdouble* __restrict__ du = dd + m;
+    const gdouble* __restrict__ dw = du + m;
 
-    // forward substitution
-    double b2 = Xc[0] / dd[0];
-    Xc[0] = b2;
-    double b1 = (Xc[s] - dl[1] * b2) / dd[1];
-    Xc[s] = b1;
+    // ---- forward substitution: rows 0, 1, then blocks of DB rows from row 2
+    double b2 = 0.0, b1 = 0.0;
+    if (live) {
+        b2 = Xc[0] / dd[0];
+        Xc[0] = b2;
+        b1 = (Xc[s] - dl[1] * b2) / dd[1];
+        Xc[s] = b1;
+    }
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;  // lane's rows of the next diagonal block
+    {
+        const int rr = 2 + lane;
+        if (rr < m) {
+            sd[0][0][lane] = ds[rr];
+            sd[0][1][lane] = dl[rr];
+            sd[0][2][lane] = dd[rr];
+        }
+        const int rn = 2 + DB + lane;
+        if (rn < m) {
+            p0 = ds[rn];
+            p1 = dl[rn];
+            p2 = dd[rn];
+        }
+    }
+    __syncthreads();
     double cur[PF], nxt[PF];
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (2 + k < m) ? Xc[(2 + k) * s] : 0.0;
+    for (int k = 0; k < PF; k++) cur[k] = (live && 2 + k < m) ? Xc[(2 + k) * s] : 0.0;
     for (int i0 = 2; i0 < m; i0 += PF) {
+        const int rel = i0 - 2;
+        const int blk = rel / DB;
+        if (rel > 0 && rel % DB == 0) {
+            // block blk starts: its diagonals (prefetched during block blk-1)
+            // go to LDS buffer blk & 1, then block blk+1 is prefetched
+            sd[blk & 1][0][lane] = p0;
+            sd[blk & 1][1][lane] = p1;
+            sd[blk & 1][2][lane] = p2;
+            const int rn = 2 + (blk + 1) * DB + lane;
+            if (rn < m) {
+                p0 = ds[rn];
+                p1 = dl[rn];
+                p2 = dd[rn];
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int i = i0 + PF + k;
-            nxt[k] = (i < m) ? Xc[i * s] : 0.0;
+            nxt[k] = (live && i < m) ? Xc[i * s] : 0.0;
         }
+        const double(*dg)[DB] = sd[blk & 1];
+        const int o = rel - blk * DB;
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int i = i0 + k;
             if (i < m) {
-                const double t = cur[k] - ds[i] * b2;
-                const double v = (t - dl[i] * b1) / dd[i];
-                Xc[i * s] = v;
+                const double t = cur[k] - dg[0][o + k] * b2;
+                const double v = (t - dg[1][o + k] * b1) / dg[2][o + k];
+                if (live) Xc[i * s] = v;
                 b2 = b1;
                 b1 = v;
             }
@@ -210,27 +257,58 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
 #pragma unroll
         for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
-    // backward substitution: b1 = x[m-1], b2 = b[m-2]
+
+    // ---- backward substitution: b1 = x[m-1], b2 = b[m-2]; blocks of DB rows
+    // descending from row m-3
     const double xm1 = b1;
     double a1 = b1;
     double a0 = b2 - du[m - 2] * a1;
-    Xc[(m - 2) * s] = a0;
+    if (live) Xc[(m - 2) * s] = a0;
     const double xm2 = a0;
+    __syncthreads();  // every read of the forward blocks is done
+    {
+        const int rr = m - 3 - lane;
+        if (rr >= 0) {
+            sd[0][0][lane] = du[rr];
+            sd[0][1][lane] = dw[rr];
+        }
+        const int rn = m - 3 - DB - lane;
+        p0 = p1 = 0.0;
+        if (rn >= 0) {
+            p0 = du[rn];
+            p1 = dw[rn];
+        }
+    }
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (m - 3 - k >= 0) ? Xc[(m - 3 - k) * s] : 0.0;
+    for (int k = 0; k < PF; k++) cur[k] = (live && m - 3 - k >= 0) ? Xc[(m - 3 - k) * s] : 0.0;
     for (int i0 = m - 3; i0 >= 0; i0 -= PF) {
+        const int rel = m - 3 - i0;
+        const int blk = rel / DB;
+        if (rel > 0 && rel % DB == 0) {
+            sd[blk & 1][0][lane] = p0;
+            sd[blk & 1][1][lane] = p1;
+            const int rn = m - 3 - (blk + 1) * DB - lane;
+            if (rn >= 0) {
+                p0 = du[rn];
+                p1 = dw[rn];
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int i = i0 - PF - k;
-            nxt[k] = (i >= 0) ? Xc[i * s] : 0.0;
+            nxt[k] = (live && i >= 0) ? Xc[i * s] : 0.0;
         }
+        const double(*dg)[DB] = sd[blk & 1];
+        const int o = rel - blk * DB;
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int i = i0 - k;
             if (i >= 0) {
-                const double t = cur[k] - du[i] * a0;
-                const double v = t - dw[i] * a1;
-                Xc[i * s] = v;
+                const double t = cur[k] - dg[0][o + k] * a0;
+                const double v = t - dg[1][o + k] * a1;
+                if (live) Xc[i * s] = v;
                 a1 = a0;
                 a0 = v;
             }
@@ -238,6 +316,7 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
 #pragma unroll
         for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
+    if (!live) return;
     // solveEnd: a0 = x[0], a1 = x[1]
     const double nx2 = Xc[(n - 2) * s] - (D.e * a0 + D.a * xm2 + D.b * xm1);
     const double nx1 = Xc[(n - 1) * s] - (D.d * a0 + D.e * a1 + D.a * xm1);
