@@ -23,6 +23,11 @@
 namespace custen {
 namespace chdev {
 
+// Global-address-space views of the descriptors' field pointers: accesses
+// through them are global (not flat) instructions, which the wait counters
+// track separately from LDS traffic.
+using gdouble = __attribute__((address_space(1))) double;
+
 __device__ __forceinline__ int findRequest(const CHDesc* d, int count, int item, int which)
 {
     // largest r with base(r) <= item; bases ascend with r
@@ -60,8 +65,9 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
     const CHDesc& D = descs[r];
     const int n = D.n;
     const int par = (step - D.admitStep) & 1;
-    const double* __restrict__ c = D.f[par];
-    const double* __restrict__ co = D.f[par ^ 1];
+    const gdouble* __restrict__ c = (const gdouble*)D.f[par];
+    const gdouble* __restrict__ co = (const gdouble*)D.f[par ^ 1];
+    gdouble* __restrict__ Xo = (gdouble*)D.X;
     const int local = item - D.tileBase;
     const int tilesX = (n + RX - 1) / RX;
     const int x0 = (local % tilesX) * RX;
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
         const double t1 = c[k] - co[k];
         const double t2 = k23 * t1;
         const double t3 = t2 + nl;
-        D.X[k] = lin + t3;
+        Xo[k] = lin + t3;
     }
 }
 
@@ -128,13 +134,13 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
     const int local = item - D.trBase;
     const int R0 = (local / tiles) * kCHTr;  // source rows
     const int C0 = (local % tiles) * kCHTr;  // source columns
-    const double* __restrict__ src = corr ? D.Y : D.X;
-    double* __restrict__ dst = corr ? D.X : D.Y;
+    const gdouble* __restrict__ src = (const gdouble*)(corr ? D.Y : D.X);
+    gdouble* __restrict__ dst = (gdouble*)(corr ? D.X : D.Y);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = C0 + lane;
     double e2 = 0.0, e1 = 0.0;
-    const double* inv1 = D.fac + 5 * m;
-    const double* inv2 = inv1 + m;
+    const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
+    const gdouble* inv2 = inv1 + m;
     if (corr && c < n) {
         e2 = src[(int64_t)(n - 2) * n + c];
         e1 = src[(int64_t)(n - 1) * n + c];
@@ -165,10 +171,6 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 // ---------------------------------------------------------------------------
 constexpr int PF = 16;  // rows of the column in registers ahead of the recurrence
 constexpr int DB = 64;  // rows of the shared diagonals staged in LDS per block
-
-// Global-address-space views of descriptor pointers: loads through them are
-// global (not flat) loads, which the waitcnt logic tracks separately from LDS.
-using gdouble = __attribute__((address_space(1))) double;
 
 __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
                                                      int which)
@@ -335,11 +337,11 @@ __global__ __launch_bounds__(256) void finalKernel(const CHDesc* __restrict__ de
     const CHDesc& D = descs[r];
     const int n = D.n, m = n - 2;
     const int par = (step - D.admitStep) & 1;
-    const double* __restrict__ c = D.f[par];
-    double* __restrict__ co = D.f[par ^ 1];
-    const double* __restrict__ X = D.X;
-    const double* inv1 = D.fac + 5 * m;
-    const double* inv2 = inv1 + m;
+    const gdouble* __restrict__ c = (const gdouble*)D.f[par];
+    gdouble* __restrict__ co = (gdouble*)D.f[par ^ 1];
+    const gdouble* __restrict__ X = (const gdouble*)D.X;
+    const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
+    const gdouble* inv2 = inv1 + m;
     const int local = item - D.tileBase;
     const int tilesX = (n + RX - 1) / RX;
     const int x0 = (local % tilesX) * RX;

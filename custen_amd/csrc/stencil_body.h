@@ -23,8 +23,8 @@ __device__ __forceinline__ void stencilStripImpl(const StencilLaunch& a, const T
     constexpr int TW = kLanes * CPL;
     constexpr int LD = ldsRowLengthW(TW, H);
 
-    const T* __restrict__ in = static_cast<const T*>(a.in);
-    T* __restrict__ out = static_cast<T*>(a.out);
+    const Glob<T>* __restrict__ in = globalPtr<T>(a.in);
+    Glob<T>* __restrict__ out = globalPtr<T>(a.out);
 
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);

@@ -90,6 +90,23 @@ __host__ __device__ constexpr int ldsRowLength(int H) { return ldsRowLengthW(kTW
 
 // The per-lane element group of a strip row: one value (CPL = 1) or an
 // aligned pair (CPL = 2).
+// Global-memory (address space 1) views.  Loads and stores through them are
+// global rather than flat instructions even when the pointer was read from a
+// table in memory (batched grids); flat operations would also count against
+// the LDS wait counter, so every LDS wait would drain the row prefetch.
+template <typename T> using Glob = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ const Glob<T>* globalPtr(const void* p)
+{
+    return (const Glob<T>*)p;
+}
+template <typename T>
+__device__ __forceinline__ Glob<T>* globalPtr(void* p)
+{
+    return (Glob<T>*)p;
+}
+
 template <typename T, int CPL> struct LaneVec;
 template <typename T> struct LaneVec<T, 2> {
     using type = typename Vec2<T>::type;
@@ -176,7 +193,7 @@ __device__ __forceinline__ int mapRowNear(const RowMap& m, int gy)
 // halo-column indices need at most one wrap (ny >= window height, nx >= 128
 // > any extent): no division in the address arithmetic.
 template <typename T, int NR, bool FAST, int CPL = 2>
-__device__ __forceinline__ void loadRows(StagedRows<T, NR, CPL>& s, const T* __restrict__ in,
+__device__ __forceinline__ void loadRows(StagedRows<T, NR, CPL>& s, const Glob<T>* __restrict__ in,
                                          int64_t ldIn, const RowMap& rows, int y0, int T_, int x0,
                                          int nx, int L, int R, int xWrap, bool vecIn, int wave,
                                          int lane)
@@ -189,10 +206,10 @@ __device__ __forceinline__ void loadRows(StagedRows<T, NR, CPL>& s, const T* __r
         const int r = wave + i * kWaves;
         if (r < NR) {
             const int sy = FAST ? mapRowNear(rows, y0 - T_ + r) : mapRow(rows, y0 - T_ + r);
-            const T* grow = in + (int64_t)sy * ldIn;
+            const Glob<T>* grow = in + (int64_t)sy * ldIn;
             const int gx = x0 + CPL * lane;
             if (FAST || vecIn) {
-                s.body[i] = *reinterpret_cast<const VT*>(grow + gx);
+                s.body[i] = *reinterpret_cast<const Glob<VT>*>(grow + gx);
             } else {
 #pragma unroll
                 for (int e = 0; e < CPL; e++) LV::set(s.body[i], e, grow[mapCol(gx + e, nx, xWrap)]);
@@ -259,8 +276,9 @@ __device__ __forceinline__ void storeRows(const StagedRows<T, NR, CPL>& s, T* __
 
 // Write one output row's CPL values for this lane, honouring the write window.
 template <typename T, int CPL>
-__device__ __forceinline__ void storeLane(T* __restrict__ orow, int gx, const T* v, bool fullRow,
-                                          int nx, int wx0, int wx1, int zeroRight, int R)
+__device__ __forceinline__ void storeLane(Glob<T>* __restrict__ orow, int gx, const T* v,
+                                          bool fullRow, int nx, int wx0, int wx1, int zeroRight,
+                                          int R)
 {
     using LV = LaneVec<T, CPL>;
     using VT = typename LV::type;
@@ -268,7 +286,7 @@ __device__ __forceinline__ void storeLane(T* __restrict__ orow, int gx, const T*
         VT o;
 #pragma unroll
         for (int e = 0; e < CPL; e++) LV::set(o, e, v[e]);
-        __builtin_nontemporal_store(o, reinterpret_cast<VT*>(orow + gx));
+        __builtin_nontemporal_store(o, reinterpret_cast<Glob<VT>*>(orow + gx));
         return;
     }
 #pragma unroll
