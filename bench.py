@@ -189,7 +189,7 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
     stream = torch.cuda.current_stream()
     timer = Timer(torch, stream)
     inner = (slice(2, 258), slice(2, n - 2))
-    k2 = max(5, args.steps // 4)
+    k2 = max(20, args.steps // 2)
     wl = torch.from_numpy(workloads.laplacian9_weights(n)).cuda()
     fptr = api.custenGetBuiltinFunction("xy_cube_minus", 0)
     for kind, w_ in (("xy_np_bih", wb), ("xy_p_lap", wl), ("xy_np_lap", wl),
