@@ -747,17 +747,68 @@ extern "C" size_t custenCHFootprint(int n) { return n >= 8 ? footprintBytes(n) :
 // ===========================================================================
 // Server
 // ===========================================================================
+// Requests of one grid size share a lane: one batched engine on its own
+// stream.  Lanes advance independently (a lane of small grids is not held to
+// the pace of a lane of large ones), several steps per enqueue; the host only
+// waits when no lane can take more work.
+struct CHLane {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::unique_ptr<CHEngine> engine;
+    int inFlight = 0;            // steps enqueued and not yet accounted
+    Clock::time_point enqueued;  // when the in-flight batch was enqueued
+};
+
 struct custenCHServer {
     int device = 0;
     size_t budget = 0;
     int maxSlots = 0;
+    int resident = 0;
     size_t used = 0;
-    hipStream_t stream = nullptr;
-    std::unique_ptr<CHEngine> engine;
+    std::map<int, std::unique_ptr<CHLane>> lanes;
     std::unique_ptr<BufferPool> pool;
     std::deque<CHRequest*> queue;
     std::vector<std::unique_ptr<CHRequest>> all;
 };
+
+namespace {
+
+// Steps a lane enqueues at once (fewer if a request finishes earlier).
+constexpr int kLaneBatch = 8;
+
+void releaseLane(CHLane& l)
+{
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    l.engine.reset();
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+    l.done = nullptr;
+    l.stream = nullptr;
+}
+
+hipError_t laneFor(custenCHServer* s, int n, CHLane*& out)
+{
+    auto it = s->lanes.find(n);
+    if (it != s->lanes.end()) {
+        out = it->second.get();
+        return hipSuccess;
+    }
+    std::unique_ptr<CHLane> l(new CHLane());
+    l->n = n;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&l->done, hipEventDisableTiming)) != hipSuccess) {
+        releaseLane(*l);
+        return e;
+    }
+    l->engine.reset(new CHEngine(s->device));
+    out = l.get();
+    s->lanes[n] = std::move(l);
+    return hipSuccess;
+}
+
+}  // namespace
 
 extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetBytes, int maxSlots)
 {
@@ -774,11 +825,6 @@ extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetB
     s->device = deviceNum;
     s->budget = hbmBudgetBytes;
     s->maxSlots = maxSlots;
-    if ((e = hipStreamCreate(&s->stream)) != hipSuccess) {
-        setHipError(e, "custenCHServerCreate");
-        return nullptr;
-    }
-    s->engine.reset(new CHEngine(deviceNum));
     s->pool.reset(new BufferPool(hbmBudgetBytes / 4));
     return s.release();
 }
@@ -821,64 +867,136 @@ extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* par
 
 namespace {
 
+// Account a lane's finished batch: every resident request advanced by the
+// batch's steps; finished ones are copied out and retired.
+hipError_t retireLane(custenCHServer* s, CHLane& l)
+{
+    hipError_t e;
+    std::vector<CHRequest*> fin;
+    for (CHRequest* r : l.engine->active()) {
+        r->done += l.inFlight;
+        if (r->done >= r->steps) fin.push_back(r);
+    }
+    l.inFlight = 0;
+    for (CHRequest* r : fin) {
+        const size_t nn = (size_t)r->co.n * r->co.n;
+        r->result.resize(nn);
+        if ((e = hipMemcpyAsync(r->result.data(), l.engine->current(r), nn * sizeof(double),
+                                hipMemcpyDeviceToHost, l.stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(l.stream)) != hipSuccess)
+            return e;
+        r->tFinish = Clock::now();
+        r->state = 2;
+        l.engine->remove(r);
+        s->pool->put(r->buf);
+        r->buf = CHBuffers{};
+        s->used -= footprintBytes(r->co.n);
+        s->resident--;
+        std::vector<double>().swap(r->c0);
+        std::vector<double>().swap(r->cOld0);
+    }
+    return hipSuccess;
+}
+
+// One scheduling round: retire the work of every lane whose batch finished,
+// admit queued requests (FIFO, while a slot and the HBM budget allow and the
+// request's lane is idle), enqueue the next batch of steps on every idle lane
+// with resident requests.  If no lane was idle, first wait for the batch that
+// was enqueued earliest.
 int serverIteration(custenCHServer* s)
 {
     (void)hipSetDevice(s->device);
     hipError_t e;
+    auto idle = [](CHLane& l) {
+        if (l.inFlight == 0) return true;
+        const hipError_t q = hipEventQuery(l.done);
+        if (q == hipErrorNotReady) return false;
+        return true;  // finished, or an error that the next call reports
+    };
+    // can this round do anything without waiting?
+    bool canWork = false;
+    bool anyInFlight = false;
+    for (auto& kv : s->lanes) {
+        CHLane& l = *kv.second;
+        anyInFlight = anyInFlight || l.inFlight > 0;
+        if (l.inFlight > 0 ? idle(l) : !l.engine->active().empty()) canWork = true;
+    }
+    if (!canWork && !s->queue.empty() && s->resident < s->maxSlots) {
+        const CHRequest* r = s->queue.front();
+        auto it = s->lanes.find(r->co.n);
+        if (s->used + footprintBytes(r->co.n) <= s->budget &&
+            (it == s->lanes.end() || it->second->inFlight == 0))
+            canWork = true;
+    }
+    if (!canWork && anyInFlight) {
+        CHLane* first = nullptr;
+        for (auto& kv : s->lanes)
+            if (kv.second->inFlight > 0 && (!first || kv.second->enqueued < first->enqueued))
+                first = kv.second.get();
+        if (first && (e = hipEventSynchronize(first->done)) != hipSuccess) {
+            setHipError(e, "custenCHServerStep");
+            return -1;
+        }
+    }
+    for (auto& kv : s->lanes) {
+        CHLane& l = *kv.second;
+        if (l.inFlight == 0 || !idle(l)) continue;
+        if ((e = hipEventSynchronize(l.done)) != hipSuccess ||
+            (e = retireLane(s, l)) != hipSuccess) {
+            setHipError(e, "custenCHServerStep (retire)");
+            return -1;
+        }
+    }
     // admission: FIFO while a slot and the HBM budget allow
-    while (!s->queue.empty() && (int)s->engine->active().size() < s->maxSlots) {
+    while (!s->queue.empty() && s->resident < s->maxSlots) {
         CHRequest* r = s->queue.front();
         const size_t fp = footprintBytes(r->co.n);
         if (s->used + fp > s->budget) break;
+        CHLane* l = nullptr;
+        if ((e = laneFor(s, r->co.n, l)) != hipSuccess) {
+            setHipError(e, "custenCHServerStep (lane)");
+            return -1;
+        }
+        if (l->inFlight > 0) break;  // its lane is mid-batch: admit at the next round
         if ((e = s->pool->get(r->co.n, r->buf)) != hipSuccess) {
-            if (s->engine->active().empty()) {
+            if (s->resident == 0) {
                 setHipError(e, "custenCHServerStep (allocation)");
                 return -1;
             }
             break;  // wait for memory to free up
         }
-        if ((e = s->engine->admit(r, r->c0.data(), r->cOld0.empty() ? nullptr : r->cOld0.data(),
-                                  s->stream)) != hipSuccess) {
+        if ((e = l->engine->admit(r, r->c0.data(), r->cOld0.empty() ? nullptr : r->cOld0.data(),
+                                  l->stream)) != hipSuccess) {
             setHipError(e, "custenCHServerStep (admit)");
             return -1;
         }
         s->used += fp;
+        s->resident++;
         r->state = 1;
         r->tAdmit = Clock::now();
         s->queue.pop_front();
     }
-    if (!s->engine->active().empty()) {
-        if ((e = s->engine->stepAll(s->stream, false)) != hipSuccess ||
-            (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+    // enqueue the next batch on every idle lane
+    for (auto& kv : s->lanes) {
+        CHLane& l = *kv.second;
+        if (l.inFlight > 0 || l.engine->active().empty()) continue;
+        int k = kLaneBatch;
+        for (CHRequest* r : l.engine->active()) k = std::min(k, r->steps - r->done);
+        k = std::max(k, 1);
+        for (int i = 0; i < k; i++) {
+            if ((e = l.engine->stepAll(l.stream, false)) != hipSuccess) {
+                setHipError(e, "custenCHServerStep");
+                return -1;
+            }
+        }
+        if ((e = hipEventRecord(l.done, l.stream)) != hipSuccess) {
             setHipError(e, "custenCHServerStep");
             return -1;
         }
-        // retire finished requests
-        std::vector<CHRequest*> fin;
-        for (CHRequest* r : s->engine->active()) {
-            r->done++;
-            if (r->done >= r->steps) fin.push_back(r);
-        }
-        for (CHRequest* r : fin) {
-            const size_t nn = (size_t)r->co.n * r->co.n;
-            r->result.resize(nn);
-            e = hipMemcpy(r->result.data(), s->engine->current(r), nn * sizeof(double),
-                          hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                setHipError(e, "custenCHServerStep (result)");
-                return -1;
-            }
-            r->tFinish = Clock::now();
-            r->state = 2;
-            s->engine->remove(r);
-            s->pool->put(r->buf);
-            r->buf = CHBuffers{};
-            s->used -= footprintBytes(r->co.n);
-            std::vector<double>().swap(r->c0);
-            std::vector<double>().swap(r->cOld0);
-        }
+        l.inFlight = k;
+        l.enqueued = Clock::now();
     }
-    return (int)(s->queue.size() + s->engine->active().size());
+    return (int)s->queue.size() + s->resident;
 }
 
 }  // namespace
@@ -948,13 +1066,16 @@ extern "C" void custenCHServerDestroy(custenCHServer* s)
 {
     if (s == nullptr) return;
     (void)hipSetDevice(s->device);
-    (void)hipStreamSynchronize(s->stream);
-    for (CHRequest* r : std::vector<CHRequest*>(s->engine->active())) {
-        s->engine->remove(r);
-        s->pool->put(r->buf);
+    for (auto& kv : s->lanes) {
+        CHLane& l = *kv.second;
+        (void)hipStreamSynchronize(l.stream);
+        for (CHRequest* r : std::vector<CHRequest*>(l.engine->active())) {
+            l.engine->remove(r);
+            s->pool->put(r->buf);
+        }
+        releaseLane(l);
     }
-    s->engine.reset();
+    s->lanes.clear();
     s->pool.reset();
-    (void)hipStreamDestroy(s->stream);
     delete s;
 }

@@ -172,6 +172,58 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 constexpr int PF = 16;  // rows of the column in registers ahead of the recurrence
 constexpr int DB = 64;  // rows of the shared diagonals staged in LDS per block
 
+// One PF-row block of the forward sweep (rows i0 .. i0+PF-1, guarded against
+// m when !FULL): diagonals from LDS into registers first, then the recurrence
+// x_i = ((b_i - ds_i x_{i-2}) - dl_i x_{i-1}) / d_i.
+template <bool FULL>
+__device__ __forceinline__ void forwardBlock(gdouble* __restrict__ base, int64_t s, int col, int i0,
+                                             int m, const double (*dg)[DB], int o,
+                                             const double (&cur)[PF], double& b2, double& b1)
+{
+    double a[PF], l[PF], d[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        a[k] = dg[0][o + k];
+        l[k] = dg[1][o + k];
+        d[k] = dg[2][o + k];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        if (FULL || i0 + k < m) {
+            const double t = cur[k] - a[k] * b2;
+            const double v = (t - l[k] * b1) / d[k];
+            base[(int64_t)(i0 + k) * s + col] = v;
+            b2 = b1;
+            b1 = v;
+        }
+    }
+}
+
+// One PF-row block of the backward sweep (rows i0, i0-1, ..., i0-PF+1):
+// x_i = (y_i - du_i x_{i+1}) - dw_i x_{i+2}.
+template <bool FULL>
+__device__ __forceinline__ void backwardBlock(gdouble* __restrict__ base, int64_t s, int col, int i0,
+                                              const double (*dg)[DB], int o,
+                                              const double (&cur)[PF], double& a0, double& a1)
+{
+    double u[PF], w[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        u[k] = dg[0][o + k];
+        w[k] = dg[1][o + k];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        if (FULL || i0 - k >= 0) {
+            const double t = cur[k] - u[k] * a0;
+            const double v = t - w[k] * a1;
+            base[(int64_t)(i0 - k) * s + col] = v;
+            a1 = a0;
+            a0 = v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
                                                      int which)
 {
@@ -184,9 +236,12 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     const CHDesc& D = descs[r];
     const int n = D.n, m = n - 2;
     const int lane = (int)threadIdx.x;
-    const int col = (item - D.colBase) * kCHColBlock + lane;
-    const bool live = col < n;
-    gdouble* __restrict__ Xc = (gdouble*)((which ? D.X : D.Y) + (live ? col : 0));
+    // Lanes past the last column repeat the last column: they read the same
+    // values and store the same bits, which keeps every access unconditional.
+    const int colRaw = (item - D.colBase) * kCHColBlock + lane;
+    const bool live = colRaw < n;
+    const int col = live ? colRaw : n - 1;
+    gdouble* __restrict__ base = (gdouble*)(which ? D.X : D.Y);
     const int64_t s = n;
     const gdouble* __restrict__ ds = (const gdouble*)D.fac;
     const gdouble* __restrict__ dl = ds + m;
@@ -195,13 +250,10 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     const gdouble* __restrict__ dw = du + m;
 
     // ---- forward substitution: rows 0, 1, then blocks of DB rows from row 2
-    double b2 = 0.0, b1 = 0.0;
-    if (live) {
-        b2 = Xc[0] / dd[0];
-        Xc[0] = b2;
-        b1 = (Xc[s] - dl[1] * b2) / dd[1];
-        Xc[s] = b1;
-    }
+    double b2 = base[col] / dd[0];
+    base[col] = b2;
+    double b1 = (base[s + col] - dl[1] * b2) / dd[1];
+    base[s + col] = b1;
     double p0 = 0.0, p1 = 0.0, p2 = 0.0;  // lane's rows of the next diagonal block
     {
         const int rr = 2 + lane;
@@ -220,7 +272,7 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     __syncthreads();
     double cur[PF], nxt[PF];
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (live && 2 + k < m) ? Xc[(2 + k) * s] : 0.0;
+    for (int k = 0; k < PF; k++) cur[k] = (2 + k < m) ? base[(int64_t)(2 + k) * s + col] : 0.0;
     for (int i0 = 2; i0 < m; i0 += PF) {
         const int rel = i0 - 2;
         const int blk = rel / DB;
@@ -238,24 +290,20 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
             }
             __syncthreads();
         }
+        if (i0 + 2 * PF <= m) {
 #pragma unroll
-        for (int k = 0; k < PF; k++) {
-            const int i = i0 + PF + k;
-            nxt[k] = (live && i < m) ? Xc[i * s] : 0.0;
+            for (int k = 0; k < PF; k++) nxt[k] = base[(int64_t)(i0 + PF + k) * s + col];
+        } else {
+#pragma unroll
+            for (int k = 0; k < PF; k++) {
+                const int i = i0 + PF + k;
+                nxt[k] = i < m ? base[(int64_t)i * s + col] : 0.0;
+            }
         }
         const double(*dg)[DB] = sd[blk & 1];
         const int o = rel - blk * DB;
-#pragma unroll
-        for (int k = 0; k < PF; k++) {
-            const int i = i0 + k;
-            if (i < m) {
-                const double t = cur[k] - dg[0][o + k] * b2;
-                const double v = (t - dg[1][o + k] * b1) / dg[2][o + k];
-                if (live) Xc[i * s] = v;
-                b2 = b1;
-                b1 = v;
-            }
-        }
+        if (i0 + PF <= m) forwardBlock<true>(base, s, col, i0, m, dg, o, cur, b2, b1);
+        else forwardBlock<false>(base, s, col, i0, m, dg, o, cur, b2, b1);
 #pragma unroll
         for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
@@ -265,7 +313,7 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     const double xm1 = b1;
     double a1 = b1;
     double a0 = b2 - du[m - 2] * a1;
-    if (live) Xc[(m - 2) * s] = a0;
+    base[(int64_t)(m - 2) * s + col] = a0;
     const double xm2 = a0;
     __syncthreads();  // every read of the forward blocks is done
     {
@@ -283,7 +331,7 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (live && m - 3 - k >= 0) ? Xc[(m - 3 - k) * s] : 0.0;
+    for (int k = 0; k < PF; k++) cur[k] = (m - 3 - k >= 0) ? base[(int64_t)(m - 3 - k) * s + col] : 0.0;
     for (int i0 = m - 3; i0 >= 0; i0 -= PF) {
         const int rel = m - 3 - i0;
         const int blk = rel / DB;
@@ -297,29 +345,26 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
             }
             __syncthreads();
         }
+        if (i0 - 2 * PF + 1 >= 0) {
 #pragma unroll
-        for (int k = 0; k < PF; k++) {
-            const int i = i0 - PF - k;
-            nxt[k] = (live && i >= 0) ? Xc[i * s] : 0.0;
+            for (int k = 0; k < PF; k++) nxt[k] = base[(int64_t)(i0 - PF - k) * s + col];
+        } else {
+#pragma unroll
+            for (int k = 0; k < PF; k++) {
+                const int i = i0 - PF - k;
+                nxt[k] = i >= 0 ? base[(int64_t)i * s + col] : 0.0;
+            }
         }
         const double(*dg)[DB] = sd[blk & 1];
         const int o = rel - blk * DB;
-#pragma unroll
-        for (int k = 0; k < PF; k++) {
-            const int i = i0 - k;
-            if (i >= 0) {
-                const double t = cur[k] - dg[0][o + k] * a0;
-                const double v = t - dg[1][o + k] * a1;
-                if (live) Xc[i * s] = v;
-                a1 = a0;
-                a0 = v;
-            }
-        }
+        if (i0 - PF + 1 >= 0) backwardBlock<true>(base, s, col, i0, dg, o, cur, a0, a1);
+        else backwardBlock<false>(base, s, col, i0, dg, o, cur, a0, a1);
 #pragma unroll
         for (int k = 0; k < PF; k++) cur[k] = nxt[k];
     }
     if (!live) return;
     // solveEnd: a0 = x[0], a1 = x[1]
+    gdouble* __restrict__ Xc = base + col;
     const double nx2 = Xc[(n - 2) * s] - (D.e * a0 + D.a * xm2 + D.b * xm1);
     const double nx1 = Xc[(n - 1) * s] - (D.d * a0 + D.e * a1 + D.a * xm1);
     Xc[(n - 2) * s] = D.omega[0] * nx2 + D.omega[1] * nx1;
