@@ -654,8 +654,8 @@ def main():
     ap.add_argument("--ch-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the stencil path has no multi-GPU code "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the stencil path has no multi-GPU code; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "stream":
         bench_stream(args)

@@ -1,6 +1,6 @@
 // stencil_body.h -- the register-blocked strip computation shared by the
 // single-grid kernel (stencil_kernels.hip) and the batched-grids kernel
-// (batch_kernels.hip).
+// (batch.hip).
 #pragma once
 
 #include "stencil_tile.h"
