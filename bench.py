@@ -365,9 +365,34 @@ def bench_x_1024(args, torch, api, extra):
     api.cuStenCompute2DXp(h, api.DEVICE)
     torch.cuda.synchronize()
     t = Timer(torch, stream).run(lambda: api.cuStenCompute2DXp(h, api.DEVICE), 200, 10)
+    res = {"us": round(t * 1e3, 2), "GPts/s": round(n * n / (t * 1e-3) / 1e9, 1),
+           "effective_GB/s": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}
+    # the same 100 applies captured once in a HIP graph and replayed: the
+    # per-apply host launch cost drops out
+    gs = torch.cuda.Stream()
+    api.custenSetStream(h, gs.cuda_stream)
+    with torch.cuda.stream(gs):
+        for _ in range(3):
+            api.cuStenCompute2DXp(h, api.DEVICE)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=gs):
+        for _ in range(100):
+            api.cuStenCompute2DXp(h, api.DEVICE)
+    graph.replay()
+    torch.cuda.synchronize()
+    tg = Timer(torch, gs).run(lambda: graph.replay(), 10, 2) / 100
+    res["graph_us"] = round(tg * 1e3, 2)
+    res["graph_effective_GB/s"] = round(16.0 * n * n / (tg * 1e-3) / 1e9, 1)
+    # the graph's output equals one plain apply (same input every time)
+    ref = dout.clone()
+    api.custenSetStream(h, stream.cuda_stream)
+    api.cuStenCompute2DXp(h, api.DEVICE)
+    torch.cuda.synchronize()
+    res["graph_matches_eager"] = bool(torch.equal(ref, dout))
+    del graph
     api.cuStenDestroy2DXp(h)
-    extra["x_p_5pt_1024"] = {"us": round(t * 1e3, 2), "GPts/s": round(n * n / (t * 1e-3) / 1e9, 1),
-                             "effective_GB/s": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}
+    extra["x_p_5pt_1024"] = res
 
 
 def bench_batch(args, torch, api, extra):

@@ -78,6 +78,18 @@ MemKind classify(const void* p, int device)
     }
 }
 
+MemKind classifyCached(Plan* plan, const void* p, int device)
+{
+    for (int i = 0; i < 8; i++)
+        if (plan->kindPtr[i] == p && p != nullptr) return plan->kind[i];
+    const MemKind k = classify(p, device);
+    const int slot = plan->kindNext;
+    plan->kindPtr[slot] = p;
+    plan->kind[slot] = k;
+    plan->kindNext = (slot + 1) & 7;
+    return k;
+}
+
 Plan* planOf(cuSten_t* h, const char* fn)
 {
     if (h == nullptr) {
@@ -377,7 +389,7 @@ const double* deviceParams(Plan* p, const double* src, int count, int device, hi
                            const char* fn)
 {
     if (src == nullptr || count <= 0) return src;
-    if (classify(src, device) == MemKind::Device) return src;
+    if (classifyCached(p, src, device) == MemKind::Device) return src;
     const size_t bytes = sizeof(double) * (size_t)count;
     if (p->paramCap < bytes) {
         if (p->paramDev) (void)hipFree(p->paramDev);
@@ -688,10 +700,12 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
     }
     const void* in = h->dataInput[0];
     void* out = h->dataOutput[0];
-    const bool resident = classify(in, h->deviceNum) == MemKind::Device &&
-                          classify(out, h->deviceNum) == MemKind::Device &&
-                          (!p->weno || (classify(h->uVel[0], h->deviceNum) == MemKind::Device &&
-                                        classify(h->vVel[0], h->deviceNum) == MemKind::Device));
+    const int dev = h->deviceNum;
+    const bool resident =
+        classifyCached(p, in, dev) == MemKind::Device &&
+        classifyCached(p, out, dev) == MemKind::Device &&
+        (!p->weno || (classifyCached(p, h->uVel[0], dev) == MemKind::Device &&
+                      classifyCached(p, h->vVel[0], dev) == MemKind::Device));
     hipStream_t st = p->hasUserStream ? p->userStream : h->streams[0];
     const double* src = p->weno ? nullptr : p->fun ? h->coe : h->weights;
     const int count = p->weno ? 0 : p->fun ? p->numCoe : p->H * p->V;

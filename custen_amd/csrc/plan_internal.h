@@ -49,6 +49,11 @@ struct Plan {
     hipEvent_t evComp[2] = {nullptr, nullptr};
     hipEvent_t evD2H[2] = {nullptr, nullptr};
     bool streamEventsReady = false;
+    // pointer -> memory kind, so that repeated Computes on the same buffers
+    // skip the runtime's pointer query
+    const void* kindPtr[8] = {};
+    MemKind kind[8] = {};
+    int kindNext = 0;
     uint32_t magic = 0x43535450u;  // "CSTP"
 };
 
@@ -59,6 +64,8 @@ int lastError();
 
 // Classify a pointer; clears any HIP error the query leaves behind.
 MemKind classify(const void* p, int device);
+// The same through the plan's small cache of recently classified pointers.
+MemKind classifyCached(Plan* plan, const void* p, int device);
 
 // Validated access to a handle's plan (records an error and returns null).
 Plan* planOf(cuSten_t* h, const char* fn);
