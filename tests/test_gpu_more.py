@@ -179,3 +179,37 @@ def test_autotuner_cache_file(gpu, tmp_path):
     assert r2.returncode == 0, r2.stderr[-2000:]
     assert r1.stdout.strip() == r2.stdout.strip()
     assert len(cache.read_text().strip().splitlines()) == 1  # reused, not re-tuned
+
+
+def test_compute_captured_in_a_graph(gpu):
+    """Compute launched into a caller's HIP graph (via custenSetStream on the
+    capturing stream): every replay, with eager work in between that changes
+    the input, matches the oracle on the input of that moment."""
+    torch = gpu
+    n = 512
+    u0 = synth.field(n, n, seed=71)
+    w = synth.uniform(25, seed=72, a=-1.0, b=1.0)
+    din = torch.from_numpy(u0.copy()).cuda()
+    dout = torch.zeros_like(din)
+    dw = torch.from_numpy(w).cuda()
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYp(h, 0, 1, n, n, 32, 32, dout, din, dw, 5, 2, 2, 5, 2, 2)
+    s = torch.cuda.Stream()
+    api.custenSetStream(h, s.cuda_stream)
+    with torch.cuda.stream(s):
+        api.cuStenCompute2DXYp(h, api.DEVICE)  # tunes / warms up outside the capture
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        api.cuStenCompute2DXYp(h, api.DEVICE)
+    for rep in range(3):
+        din.mul_(1.5).add_(0.25)  # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        cur = din.cpu().numpy()
+        ref = oracle.apply(cur, w, 5, 2, 5, 2, True)
+        got = dout.cpu().numpy()
+        assert bits_equal(got, ref), (rep, mismatches(got, ref))
+    del graph
+    api.cuStenDestroy2DXYp(h)
