@@ -35,7 +35,9 @@ namespace dev {
 template <typename T, int H, int V, int RPW, int CPL>
 struct FixedOccupancy {
     static constexpr int value =
-        (sizeof(T) == 8 && H * V >= 25 && RPW == 4 && CPL == 2) ? CUSTEN_OCC_F64_LARGE : 1;
+        (sizeof(T) == 8 && H * V >= 49) ? 2
+        : (sizeof(T) == 8 && H * V >= 25 && RPW == 4 && CPL == 2) ? CUSTEN_OCC_F64_LARGE
+                                                                  : 1;
 };
 
 template <typename T, int H, int V, int RPW, Xform X, bool FAST, int CPL>
@@ -384,6 +386,8 @@ hipError_t runGeneric(const StencilLaunch& s, hipStream_t st)
 // xy: 3x3, 5x5.  RPW in {4, 8}.
 #define CUSTEN_FIXED_SHAPES(X_) \
     X_(3, 1) X_(5, 1) X_(7, 1) X_(9, 1) X_(1, 3) X_(1, 5) X_(1, 7) X_(1, 9) X_(3, 3) X_(5, 5)
+#define CUSTEN_WIDE_SHAPES(X_) \
+    X_(11, 1) X_(13, 1) X_(1, 11) X_(1, 13) X_(7, 7) X_(9, 9) X_(3, 5) X_(5, 3)
 
 template <typename T, Xform X>
 hipError_t dispatchTyped(const StencilLaunch& s, hipStream_t st)
@@ -403,6 +407,13 @@ hipError_t dispatchTyped(const StencilLaunch& s, hipStream_t st)
     }
     CUSTEN_FIXED_SHAPES(CUSTEN_CASE)
 #undef CUSTEN_CASE
+    // wider shapes (high-order schemes), identity transform, 4 rows per wave
+    if constexpr (X == Xform::Identity) {
+#define CUSTEN_WIDE(HH, VV) \
+    if (s.H == HH && s.V == VV) return runFixed<T, HH, VV, 4, X>(s, st);
+        CUSTEN_WIDE_SHAPES(CUSTEN_WIDE)
+#undef CUSTEN_WIDE
+    }
     return runGeneric<T, X>(s, st);
 }
 

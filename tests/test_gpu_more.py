@@ -213,3 +213,23 @@ def test_compute_captured_in_a_graph(gpu):
         assert bits_equal(got, ref), (rep, mismatches(got, ref))
     del graph
     api.cuStenDestroy2DXYp(h)
+
+
+@pytest.mark.parametrize("H,L,V,T", [(7, 3, 7, 3), (9, 4, 9, 4), (3, 1, 5, 2), (13, 6, 1, 0)])
+def test_f32_wide_shapes(gpu, H, L, V, T):
+    torch = gpu
+    nx, ny = 384, 96
+    u = synth.field(nx, ny, seed=81).astype(np.float32)
+    w = synth.uniform(H * V, seed=82, a=-1.0, b=1.0)
+    din = torch.from_numpy(u).cuda()
+    dout = torch.zeros_like(din)
+    dw = torch.from_numpy(w).cuda()
+    h = api.cuSten_t()
+    api.custenCreate2DF32(h, 0, 1, nx, ny, 2 if V > 1 else 0, 1, dout, din, dw, H, L, H - 1 - L,
+                          V, T, V - 1 - T)
+    api.custenCompute2DF32(h, api.DEVICE)
+    torch.cuda.synchronize()
+    api.custenDestroy2DF32(h)
+    ref = oracle.apply(u, w.astype(np.float32), H, L, V, T, True)
+    got = dout.cpu().numpy()
+    assert bits_equal(got, ref), mismatches(got, ref)

@@ -63,6 +63,16 @@ CASES = [
     ("2DXYp", 256, 256, 4, 3, 1, 3, 1),
     ("2DXYp", 129, 77, 1, 4, 1, 2, 0),
     ("2DXYp", 300, 100, 2, 7, 3, 7, 3),
+    ("2DXYp", 256, 64, 1, 7, 1, 7, 5),
+    ("2DXYp", 300, 100, 2, 9, 4, 9, 4),
+    ("2DXYp", 200, 64, 1, 3, 1, 5, 2),
+    ("2DXYp", 200, 64, 1, 5, 2, 3, 1),
+    ("2DXYnp", 300, 100, 1, 7, 3, 7, 3),
+    ("2DXYnp", 256, 96, 1, 9, 4, 9, 4),
+    ("2DXp", 1024, 64, 1, 13, 6, 1, 0),
+    ("2DXnp", 520, 40, 2, 11, 5, 1, 0),
+    ("2DYp", 256, 128, 2, 1, 0, 13, 6),
+    ("2DYnp", 200, 120, 1, 1, 0, 11, 5),
     ("2DXYp", 64, 32, 1, 5, 0, 5, 4),
     ("2DXYp", 5, 4, 1, 5, 2, 5, 2),
     ("2DXYnp", 128, 64, 2, 5, 2, 5, 2),
