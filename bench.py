@@ -144,7 +144,9 @@ def bench_stencil_8192(args, torch, api, extra):
     wb = torch.from_numpy(wbh).cuda()
 
     h = make_handle(api, stream, "xy_p_bih", n, dout, din, wb)
-    if args.tile:
+    if args.tile.startswith("band:"):
+        api.custenSetBandConfig(h, *(int(v) for v in args.tile[5:].split(",")))
+    elif args.tile:
         api.custenSetTileConfig(h, *(int(v) for v in args.tile.split(",")))
     api.cuStenCompute2DXYp(h, api.DEVICE)  # autotunes the tile configuration once
     torch.cuda.synchronize()
@@ -158,7 +160,10 @@ def bench_stencil_8192(args, torch, api, extra):
         torch.cuda.synchronize()
     ms = timer.run(fn, args.steps, args.warmup)
     rpw, chunks, cpl = api.custenGetTileConfig(h)
-    extra["headline_tile_config"] = {"rows_per_wave": rpw, "chunks": chunks,
+    band = api.custenGetBandMode(h)
+    extra["headline_tile_config"] = {"kernel": "band walker" if band else "strip",
+                                     "rows_per_wave": rpw,
+                                     ("chunks_per_band" if band else "chunks_per_group"): chunks,
                                      "columns_per_lane": cpl}
     destroy(api, "xy_p_bih", h)
     # parity spot check of the measured output against the oracle (rows 0..255)
@@ -671,8 +676,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--tile", default="",
-                    help="rows_per_wave,chunks,columns_per_lane for the headline apply "
-                         "(default: autotuned)")
+                    help="rows_per_wave,chunks,columns_per_lane of the strip kernel, or "
+                         "band:rows_per_wave,chunks_per_band for the band walker, for the "
+                         "headline apply (default: autotuned)")
     ap.add_argument("--ch-requests", type=int, default=24)
     ap.add_argument("--ch-steps", type=int, default=200)
     ap.add_argument("--ch-slots", type=int, default=64)

@@ -158,6 +158,8 @@ _SIGS.update({
     "custenSetTileConfigEx": (None, [_H, c_int, c_int, c_int]),
     "custenGetTileConfig": (c_int, [_H, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                     ctypes.POINTER(c_int)]),
+    "custenSetBandConfig": (None, [_H, c_int, c_int]),
+    "custenGetBandMode": (c_int, [_H]),
     "custenBatchCreate": (c_void_p, [c_int, c_int, _P, _P, _P, _P, _P, c_int, c_int, c_int,
                                      c_int, c_int]),  # noqa: E501
     "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
@@ -389,6 +391,17 @@ def custenGetTileConfig(h: cuSten_t):
     lib().custenGetTileConfig(_handle(h), ctypes.byref(r), ctypes.byref(c), ctypes.byref(l))
     check("custenGetTileConfig")
     return r.value, c.value, l.value
+
+
+def custenSetBandConfig(h: cuSten_t, rowsPerWave: int, chunksPerBand: int) -> None:
+    lib().custenSetBandConfig(_handle(h), rowsPerWave, chunksPerBand)
+    check("custenSetBandConfig")
+
+
+def custenGetBandMode(h: cuSten_t) -> int:
+    r = lib().custenGetBandMode(_handle(h))
+    check("custenGetBandMode")
+    return r
 
 
 def custenSetStream(h: cuSten_t, stream: int) -> None:

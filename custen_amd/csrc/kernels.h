@@ -58,6 +58,9 @@ struct StencilLaunch {
     // tx = stripTx0 .. stripTx0 + nStripTx - 1, edge launches the listed tx
     int stripTx0, nStripTx;
     int edgeTx[4];
+    // 1: interior strips run the band walker (FP64, identity transform):
+    // one wave per rowsPerWave * chunks rows of a strip
+    int band;
 };
 
 // Point-function apply.  Same geometry as StencilLaunch; `func` is a device
@@ -78,6 +81,7 @@ struct FunLaunch {
     int wx0, wx1, wy0, wy1, yGlobal0;
     int rowsPerWave, chunks, colsPerLane;
     int vecIn;
+    int band;                // as StencilLaunch::band (weighted-sum built-ins)
 };
 
 hipError_t launchStencil(const StencilLaunch& s, hipStream_t stream);
@@ -88,6 +92,8 @@ hipError_t builtinFunctionAddress(Builtin b, void** addr);
 
 // LDS bytes the stencil kernel uses for a given shape (reported in mem_shared).
 int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave);
+// LDS bytes of one band-walker workgroup (one wave).
+int stencilBandLdsBytes(int H, int V, int rowsPerWave);
 
 // WENO5 upwind advection over a region of rows [0, ny), periodic in x: out =
 // u * Fx + v * Fy.  `in` rows are addressed through `rows` (the kernel reads

@@ -471,9 +471,18 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
         s.yGlobal0 = r.yGlobal0;
         s.zeroRight = p->zeroRight ? 1 : 0;
         s.xform = Xform::Identity;
-        s.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
-        s.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
-        s.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
+        if (p->band) {
+            s.rowsPerWave = p->rowsPerWave;
+            s.chunks = p->chunks;
+            s.colsPerLane = 2;
+            s.band = 1;
+        } else {
+            s.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
+            s.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
+            s.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
+            const bool user = p->rowsPerWave || p->chunks || p->colsPerLane;
+            s.band = !user && p->tuned && p->tunedBand ? 1 : 0;
+        }
         return launchStencil(s, st);
     }
     FunLaunch f{};
@@ -499,9 +508,18 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
     f.wy0 = wy0;
     f.wy1 = wy1;
     f.yGlobal0 = r.yGlobal0;
-    f.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
-    f.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
-    f.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
+    if (p->band) {
+        f.rowsPerWave = p->rowsPerWave;
+        f.chunks = p->chunks;
+        f.colsPerLane = 2;
+        f.band = 1;
+    } else {
+        f.rowsPerWave = p->rowsPerWave ? p->rowsPerWave : (p->tuned ? p->tunedRpw : 0);
+        f.chunks = p->chunks ? p->chunks : (p->tuned ? p->tunedChunks : 0);
+        f.colsPerLane = p->colsPerLane ? p->colsPerLane : (p->tuned ? p->tunedCpl : 0);
+        const bool user = p->rowsPerWave || p->chunks || p->colsPerLane;
+        f.band = !user && p->tuned && p->tunedBand ? 1 : 0;
+    }
     return launchFun(f, st);
 }
 
@@ -738,7 +756,7 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
         }
         // First resident Compute of a large grid: pick the tile configuration
         // (bit-identical results for every candidate; synchronous once).
-        if (!p->weno && !p->tuned && p->rowsPerWave == 0 && p->chunks == 0 &&
+        if (!p->weno && !p->tuned && !p->band && p->rowsPerWave == 0 && p->chunks == 0 &&
             p->colsPerLane == 0 &&
             tunerEnabled() &&
             (int64_t)h->nx * h->ny >= (int64_t(1) << 20) && !streamCapturing(st)) {
@@ -755,14 +773,26 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             if (!tunerLookup(k, v)) {
                 // one-column-per-lane strips (custenSetTileConfigEx) measured
                 // slower on every shape, so they are not candidates
-                static const std::vector<TuneVal> cands = {{4, 1, 2}, {4, 2, 2}, {4, 4, 2},
-                                                           {8, 1, 2}, {8, 2, 2}, {8, 4, 2},
-                                                           {8, 8, 2}, {8, 16, 2}};
+                std::vector<TuneVal> cands = {{4, 1, 2}, {4, 2, 2}, {4, 4, 2}, {8, 1, 2},
+                                              {8, 2, 2}, {8, 4, 2}, {8, 8, 2}, {8, 16, 2}};
+                // band-walker candidates (rows per wave, chunks per band):
+                // FP64 weight stencils and weighted-sum point functions
+                const bool bandOk =
+                    p->dtype == Dtype::F64 &&
+                    (!p->fun || (p->dir == Dir::XY && !p->forceIndirect &&
+                                 p->builtin == Builtin::XYWeightedSum));
+                if (bandOk) {
+                    for (const TuneVal& b :
+                         {TuneVal{4, 2, 2, 1}, TuneVal{4, 4, 2, 1}, TuneVal{4, 8, 2, 1},
+                          TuneVal{8, 1, 2, 1}, TuneVal{8, 2, 2, 1}, TuneVal{8, 4, 2, 1}})
+                        cands.push_back(b);
+                }
                 auto run = [&](const TuneVal& c) {
                     Plan tmp = *p;
                     tmp.rowsPerWave = c.rpw;
                     tmp.chunks = c.chunks;
                     tmp.colsPerLane = c.cpl;
+                    tmp.band = c.band != 0;
                     return launchRegion(h, &tmp, r, params, st);
                 };
                 e = tunerMeasure(st, run, cands, 4, v);
@@ -776,6 +806,7 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
             p->tunedRpw = v.rpw;
             p->tunedChunks = v.chunks;
             p->tunedCpl = v.cpl;
+            p->tunedBand = v.band != 0;
         }
         e = launchRegion(h, p, r, params, st);
         if (e != hipSuccess) setHipError(e, fn);
@@ -1156,9 +1187,37 @@ void custenSetTileConfigEx(cuSten_t* h, int rowsPerWave, int chunksPerGroup, int
     p->rowsPerWave = rowsPerWave;
     p->chunks = chunksPerGroup;
     p->colsPerLane = columnsPerLane;
+    p->band = false;
     if (rowsPerWave == 0 && chunksPerGroup == 0 && columnsPerLane == 0)
         p->tuned = false;  // re-tune on next Compute
     h->mem_shared = stencilLdsBytes(p->dtype, p->H, p->V, rowsPerWave);
+}
+
+void custenSetBandConfig(cuSten_t* h, int rowsPerWave, int chunksPerBand)
+{
+    Plan* p = planOf(h, "custenSetBandConfig");
+    if (!p) return;
+    if (rowsPerWave != 4 && rowsPerWave != 8) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetBandConfig: rowsPerWave must be 4 or 8");
+        return;
+    }
+    if (chunksPerBand < 1) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenSetBandConfig: chunksPerBand must be >= 1");
+        return;
+    }
+    p->rowsPerWave = rowsPerWave;
+    p->chunks = chunksPerBand;
+    p->colsPerLane = 2;
+    p->band = true;
+    h->mem_shared = stencilBandLdsBytes(p->H, p->V, rowsPerWave);
+}
+
+int custenGetBandMode(cuSten_t* h)
+{
+    Plan* p = planOf(h, "custenGetBandMode");
+    if (!p) return -CUSTEN_ERROR_INVALID_VALUE;
+    const bool user = p->rowsPerWave || p->chunks || p->colsPerLane;
+    return (p->band || (!user && p->tuned && p->tunedBand)) ? 1 : 0;
 }
 
 void custenSetTileConfig(cuSten_t* h, int rowsPerWave, int chunksPerGroup)

@@ -35,8 +35,10 @@ struct Plan {
     int rowsPerWave = 0;        // set by custenSetTileConfig[Ex] (0: autotuned)
     int chunks = 0;
     int colsPerLane = 0;
+    bool band = false;          // set by custenSetBandConfig
     bool tuned = false;         // autotuner result for resident launches
     int tunedRpw = 0, tunedChunks = 0, tunedCpl = 0;
+    bool tunedBand = false;
     // device copy of host-resident weights / coefficients
     double* paramDev = nullptr;
     size_t paramCap = 0;

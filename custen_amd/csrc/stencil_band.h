@@ -1,0 +1,216 @@
+// stencil_band.h -- the band-walker stencil kernel: one wave streams a band
+// of rows down its own 128-column strip, with no workgroup barrier.
+//
+// Rows go from HBM straight into LDS (global_load_lds: each lane's 16 bytes
+// land at LDS base + 16 * lane, so one instruction stages a 128-column FP64
+// row body; two 4-byte loads by 2*HLP lanes stage the halo columns on each
+// side).  The wave keeps a ring of S row slots,
+//     S = (D + 1) * RPW + V - 1,
+// and while it computes chunk c (RPW output rows from RPW + V - 1 input rows)
+// the rows of chunks c+1 .. c+D are in flight into the other slots, so no
+// registers are spent on staging and the V-1 rows shared by consecutive
+// chunks are never copied.  Completion is tracked with the vector-memory
+// counter, which retires loads and stores in issue order: before chunk c the
+// wave waits until no more than (loads issued after chunk c's) + (stores
+// issued after chunk c's) operations are outstanding.  A chunk whose stores
+// did not take the one-store-per-row path (write-masked or ragged rows)
+// makes the following waits drain the counter instead.
+//
+// Arithmetic is the fma chain of the other stencil kernels (row j outer,
+// column i inner, from 0.0), so every configuration gives identical bits.
+#pragma once
+
+#include "stencil_tile.h"
+
+#pragma clang fp contract(off)
+
+namespace custen {
+namespace dev {
+
+typedef __attribute__((address_space(3))) void LdsVoid;
+
+// s_waitcnt with vmcnt = N and the other counters unconstrained (gfx9
+// encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+template <int N>
+__device__ __forceinline__ void waitVmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt is a 6-bit counter");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __asm__ __volatile__("" ::: "memory");
+}
+
+// Wait for a chunk given la = chunks of loads and sa = chunks of stores
+// issued after it (la < D, sa <= D): the immediate is la*NL + sa*NS.
+template <int NL, int NS, int D, int A = 0, int B = 0>
+__device__ __forceinline__ void waitChunk(int la, int sa)
+{
+    if constexpr (A >= D) {
+        waitVmcnt<0>();
+    } else if constexpr (B > D) {
+        waitChunk<NL, NS, D, A + 1, 0>(la, sa);
+    } else {
+        if (la == A && sa == B) waitVmcnt<A * NL + B * NS>();
+        else waitChunk<NL, NS, D, A, B + 1>(la, sa);
+    }
+}
+
+__device__ __forceinline__ void ldsDma16(const Glob<double>* g, double* l)
+{
+    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 16, 0, 0);
+}
+__device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, double* l)
+{
+    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, 0);
+}
+
+// Ring geometry of a band-walker instantiation (FP64, 2 columns per lane).
+template <int H, int V, int RPW>
+struct BandShape {
+    static constexpr int D = 8 / RPW;                  // chunks in flight ahead
+    static constexpr int HLP = ((H - 1) + 1) & ~1;     // halo columns staged per side
+    static constexpr int LD = kTW + 2 * HLP;           // slot length (doubles)
+    static constexpr int S = (D + 1) * RPW + V - 1;    // slots in the ring
+    static constexpr int NL = RPW * (H > 1 ? 3 : 1);   // load instructions per chunk
+    static constexpr int LDS_BYTES = S * LD * 8;
+};
+
+// One wave per workgroup.  Wave g (after the XCD remap) walks interior strip
+// stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...).
+// Requirements (checked by the host): FP64, identity transform, the strip is
+// interior (full 128 columns, pair-aligned rows), and wrapped row indices
+// need at most one correction for every row the band touches.
+template <int H, int V, int RPW>
+__global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
+{
+    using BS = BandShape<H, V, RPW>;
+    constexpr int D = BS::D, HLP = BS::HLP, LD = BS::LD, S = BS::S, NL = BS::NL;
+    __shared__ __attribute__((aligned(16))) double ring[S * LD];
+
+    double w[H * V];
+#pragma unroll
+    for (int k = 0; k < H * V; k++) w[k] = a.weights[k];
+
+    const int lane = threadIdx.x;
+    const int g = xcdRemap(blockIdx.x, gridDim.x);
+    const int tx = a.stripTx0 + g % a.nStripTx;
+    const int band = g / a.nStripTx;
+    const int bandRows = RPW * a.chunks;
+    const int yS = band * bandRows;
+    if (yS >= a.ny) return;
+    const int yE = min(yS + bandRows, a.ny);
+    const int nChunks = (yE - yS + RPW - 1) / RPW;
+
+    const Glob<double>* __restrict__ in = globalPtr<double>(a.in);
+    Glob<double>* __restrict__ out = globalPtr<double>(a.out);
+    const int nx = a.nx;
+    const int x0 = tx * kTW;
+    const int L = a.L;
+    const int R = H - 1 - L;
+    const bool fullRow = x0 >= a.wx0 && x0 + kTW <= a.wx1;
+
+    // halo columns staged for this lane (lanes < 2*HLP, one dword each)
+    int colL = x0 - HLP + (lane >> 1);
+    int colR = x0 + kTW + (lane >> 1);
+    if (a.xWrap) {
+        if (colL < 0) colL += nx;
+        if (colR >= nx) colR -= nx;
+    } else {
+        colL = colL < 0 ? 0 : colL;
+        colR = colR >= nx ? nx - 1 : colR;
+    }
+
+    auto loadRow = [&](int rel, int slot) {
+        const int sy = mapRowNear(a.rows, yS - a.T + rel);
+        const Glob<double>* row = in + (int64_t)sy * a.ldIn;
+        double* l = ring + slot * LD;
+        ldsDma16(row + x0 + 2 * lane, l + HLP);
+        if constexpr (H > 1) {
+            if (lane < 2 * HLP) {
+                ldsDma4((const Glob<unsigned>*)(row + colL) + (lane & 1), l);
+                ldsDma4((const Glob<unsigned>*)(row + colR) + (lane & 1), l + HLP + kTW);
+            }
+        }
+    };
+    auto loadChunk = [&](int rel0, int slot0) {
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            int sl = slot0 + r;
+            if (sl >= S) sl -= S;
+            loadRow(rel0 + r, sl);
+        }
+    };
+
+    // Chunk c computes region rows [yS + c*RPW, +RPW) from relative input rows
+    // [c*RPW, c*RPW + RPW + V - 1) (relative row 0 is yS - T), held in slots
+    // (relative row) mod S.  Its loads are the rows [c*RPW + V-1, (c+1)*RPW + V-1);
+    // chunk 0 also loads rows [0, V-1) first.
+#pragma unroll
+    for (int r = 0; r < V - 1; r++) loadRow(r, r);
+    int relLoad = V - 1, slotLoad = V - 1;
+    for (int c = 0; c < D; c++) {
+        if (c < nChunks) loadChunk(relLoad, slotLoad);
+        relLoad += RPW;
+        slotLoad += RPW;
+        if (slotLoad >= S) slotLoad -= S;
+    }
+
+    int slotBase = 0;
+    unsigned inexact = 0;  // bit i: chunk c-1-i did not store one row per instruction
+    for (int c = 0; c < nChunks; c++) {
+        if (inexact & ((1u << D) - 1u)) {
+            waitVmcnt<0>();
+        } else {
+            const int la = min(c + D - 1, nChunks - 1) - c;
+            const int sa = min(c, D);
+            waitChunk<NL, RPW, D>(la, sa);
+        }
+        // chunk c + D reuses the slots of chunk c - 1's first RPW rows
+        if (c + D < nChunks) loadChunk(relLoad, slotLoad);
+        relLoad += RPW;
+        slotLoad += RPW;
+        if (slotLoad >= S) slotLoad -= S;
+
+        double acc[RPW][2];
+#pragma unroll
+        for (int r = 0; r < RPW; r++) acc[r][0] = acc[r][1] = 0.0;
+#pragma unroll
+        for (int ir = 0; ir < RPW + V - 1; ir++) {
+            int sl = slotBase + ir;
+            if (sl >= S) sl -= S;
+            const double* row = ring + sl * LD + (HLP - L) + 2 * lane;
+            double v[H + 1];
+#pragma unroll
+            for (int k = 0; k < H + 1; k++) v[k] = row[k];
+#pragma unroll
+            for (int r = 0; r < RPW; r++) {
+                const int j = ir - r;
+                if (j >= 0 && j < V) {
+#pragma unroll
+                    for (int i = 0; i < H; i++) {
+                        acc[r][0] = fmaT(w[j * H + i], v[i], acc[r][0]);
+                        acc[r][1] = fmaT(w[j * H + i], v[i + 1], acc[r][1]);
+                    }
+                }
+            }
+        }
+
+        bool exact = fullRow;
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const int gy = yS + c * RPW + r;
+            const int gyg = a.yGlobal0 + gy;
+            if (gy >= yE || gyg < a.wy0 || gyg >= a.wy1) {
+                exact = false;
+                continue;
+            }
+            storeLane<double, 2>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r], fullRow, nx,
+                                 a.wx0, a.wx1, a.zeroRight, R);
+        }
+        inexact = (inexact << 1) | (exact ? 0u : 1u);
+        slotBase += RPW;
+        if (slotBase >= S) slotBase -= S;
+    }
+}
+
+}  // namespace dev
+}  // namespace custen

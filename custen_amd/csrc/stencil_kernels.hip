@@ -14,8 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 
 #include "kernels.h"
+#include "stencil_band.h"
 #include "stencil_body.h"
 #include "stencil_tile.h"
 
@@ -358,6 +360,18 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     if (hi > lo) {
         s.stripTx0 = lo;
         s.nStripTx = hi - lo;
+        if constexpr (std::is_same<T, double>::value && X == Xform::Identity && CPL == 2 &&
+                      (RPW == 4 || RPW == 8)) {
+            // band walker: every row it touches must need at most one wrap
+            if (s.band && (!s.rows.wrap || (s.rows.rowWrap >= s.V + RPW && s.ny <= s.rows.rowWrap))) {
+                const int bandRows = RPW * s.chunks;
+                const int64_t waves = (int64_t)s.nStripTx * ((s.ny + bandRows - 1) / bandRows);
+                if (waves > 0x7fffffff) return hipErrorInvalidValue;
+                hipLaunchKernelGGL((dev::stencilBand<H, V, RPW>), dim3((unsigned)waves),
+                                   dim3(dev::kLanes), 0, st, s);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, true, CPL>),
                            dim3(s.nStripTx * tilesY), dim3(dev::kThreads), 0, st, s);
         return hipGetLastError();
@@ -426,6 +440,14 @@ int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave)
     return es * (dev::kWaves * rpw + V - 1) * dev::ldsRowLength(H);
 }
 
+int stencilBandLdsBytes(int H, int V, int rowsPerWave)
+{
+    const int rpw = rowsPerWave == 8 ? 8 : 4;
+    const int d = 8 / rpw;
+    const int hlp = H & ~1;  // H - 1 halo columns per side, rounded up to even
+    return 8 * ((d + 1) * rpw + V - 1) * (dev::kTW + 2 * hlp);
+}
+
 hipError_t launchStencil(const StencilLaunch& in, hipStream_t st)
 {
     StencilLaunch s = in;
@@ -488,6 +510,7 @@ hipError_t launchFun(const FunLaunch& in, hipStream_t st)
         s.rowsPerWave = f.rowsPerWave;
         s.chunks = f.chunks;
         s.colsPerLane = f.colsPerLane;
+        s.band = f.band;
         return launchStencil(s, st);
     }
 

@@ -1,5 +1,5 @@
 // tuner.cpp -- autotuner for the stencil tile configuration (rows per wave x
-// row chunks per workgroup).  All configurations compute bit-identical
+// row chunks per workgroup, or per band for the band walker).  All configurations compute bit-identical
 // results; the tuner times each candidate on the caller's stream with HIP
 // events and keeps the fastest per (dtype, stencil shape, grid shape) in a
 // process-wide table, optionally persisted in the file named by
@@ -38,11 +38,17 @@ void loadLocked()
     if (!path) return;
     FILE* f = std::fopen(path, "r");
     if (!f) return;
-    TuneKey k;
-    TuneVal v;
-    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d", &k.kind, &k.dtype, &k.H, &k.V,
-                       &k.xform, &k.nx, &k.ny, &k.count, &v.rpw, &v.chunks, &v.cpl) == 11)
-        g_table[k] = v;
+    // one entry per line, 12 fields; lines of any other shape are skipped
+    char line[256];
+    while (std::fgets(line, sizeof(line), f)) {
+        TuneKey k;
+        TuneVal v;
+        int extra = 0;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d", &k.kind, &k.dtype, &k.H,
+                        &k.V, &k.xform, &k.nx, &k.ny, &k.count, &v.rpw, &v.chunks, &v.cpl,
+                        &v.band, &extra) == 12)
+            g_table[k] = v;
+    }
     std::fclose(f);
 }
 
@@ -79,8 +85,8 @@ void tunerStore(const TuneKey& k, const TuneVal& v)
     if (!path) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d\n", k.kind, k.dtype, k.H, k.V, k.xform, k.nx,
-                 k.ny, k.count, v.rpw, v.chunks, v.cpl);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d\n", k.kind, k.dtype, k.H, k.V, k.xform,
+                 k.nx, k.ny, k.count, v.rpw, v.chunks, v.cpl, v.band);
     std::fclose(f);
 }
 

@@ -22,6 +22,7 @@ struct TuneVal {
     int rpw = 8;
     int chunks = 4;
     int cpl = 2;  // columns per lane
+    int band = 0;  // 1: band walker (chunks = chunks per band)
 };
 
 bool tunerEnabled();

@@ -305,6 +305,19 @@ void custenSetTileConfigEx(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGr
 int custenGetTileConfig(cuSten_t* pt_cuSten, int* rowsPerWave, int* chunksPerGroup,
                         int* columnsPerLane);
 
+/* Band walker for FP64 weight stencils (and the weighted-sum point
+ * function): each wave streams its own 128-column strip over a band of
+ * rowsPerWave * chunksPerBand rows, with rows loaded from memory directly into
+ * a ring of LDS rows, 8 rows per wave in flight ahead of the computed chunk.
+ * rowsPerWave 4 or 8, chunksPerBand >= 1.  Results are identical to the other
+ * configurations; shapes or grids the walker does not cover use the strip
+ * kernel with the same rowsPerWave and chunks.  custenSetTileConfig[Ex]
+ * switches it off again. */
+void custenSetBandConfig(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerBand);
+/* 1 if Compute uses the band walker (set by the caller or chosen by the
+ * autotuner), 0 if not, negative on error. */
+int custenGetBandMode(cuSten_t* pt_cuSten);
+
 /* WENO handles: 1 (default) squares the smoothness terms in single precision
  * as the reference does, 0 squares them in double. */
 void custenSetWenoSquares(cuSten_t* pt_cuSten, int singlePrecision);

@@ -85,6 +85,14 @@ def test_uncreated_handle_is_rejected():
         api.cuStenDestroy2DXYp(h)
 
 
+def test_tile_and_band_config_need_a_handle():
+    h = api.cuSten_t()
+    for call in (lambda: api.custenSetBandConfig(h, 4, 2), lambda: api.custenGetBandMode(h),
+                 lambda: api.custenSetTileConfig(h, 4, 2)):
+        with pytest.raises(api.CustenError):
+            call()
+
+
 def test_builtin_names():
     with pytest.raises(api.CustenError):
         api.custenGetBuiltinFunction("no_such_function", 0)

@@ -1,0 +1,156 @@
+"""GPU parity of the band walker (custenSetBandConfig) against the oracle.
+
+The walker covers the interior 128-column strips of FP64 weight stencils; edge
+strips, other dtypes and shapes run the strip kernel.  Every case here is
+compared bit for bit with the oracle, with ragged widths and heights so that
+bands end inside a chunk, np write masks fall inside walker strips, and
+chunks of stores that are not one-store-per-row occur next to full ones."""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle
+from custen_amd import api, synth
+from tests.helpers import bits_equal, mismatches
+from tests.test_gpu_stencil import SENTINEL, create_weights, dev, oracle_weights
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    # variant, H, L, V, T
+    ("2DXp", 3, 1, 1, 0), ("2DXp", 5, 2, 1, 0), ("2DXnp", 7, 3, 1, 0), ("2DXp", 9, 4, 1, 0),
+    ("2DXnp", 11, 5, 1, 0), ("2DXp", 13, 6, 1, 0),
+    ("2DYp", 1, 0, 3, 1), ("2DYnp", 1, 0, 5, 2), ("2DYp", 1, 0, 7, 3), ("2DYp", 1, 0, 9, 4),
+    ("2DYp", 1, 0, 11, 5), ("2DYnp", 1, 0, 13, 6),
+    ("2DXYp", 3, 1, 3, 1), ("2DXYnp", 3, 1, 3, 1), ("2DXYp", 5, 2, 5, 2), ("2DXYnp", 5, 2, 5, 2),
+    ("2DXYp", 5, 0, 5, 4), ("2DXYp", 3, 1, 5, 2), ("2DXYp", 5, 2, 3, 1), ("2DXYp", 7, 3, 7, 3),
+    ("2DXYnp", 9, 4, 9, 4),
+]
+
+
+def _run(torch, variant, nx, ny, H, L, V, T, rpw, chunks, seed_extra=0):
+    seed = zlib.crc32(repr((variant, nx, ny, H, L, V, T, seed_extra)).encode()) & 0xFFFF
+    u = synth.field(nx, ny, seed)
+    w = synth.uniform(H * V, seed + 1, -2.0, 2.0)
+    init = np.full_like(u, SENTINEL)
+    du, dout, dw = dev(torch, u), dev(torch, init), dev(torch, w)
+    h = api.cuSten_t()
+    create_weights(variant, h, 1, nx, ny, dout, du, dw, H, L, V, T)
+    api.custenSetBandConfig(h, rpw, chunks)
+    assert api.custenGetBandMode(h) == 1
+    getattr(api, f"cuStenCompute{variant}")(h, api.DEVICE)
+    torch.cuda.synchronize()
+    getattr(api, f"cuStenDestroy{variant}")(h)
+    return dout.cpu().numpy(), oracle_weights(variant, u, w, H, L, V, T, init)
+
+
+@pytest.mark.parametrize("variant,H,L,V,T", SHAPES)
+@pytest.mark.parametrize("rpw,chunks", [(4, 1), (4, 3), (8, 2)])
+def test_band_shapes(gpu, variant, H, L, V, T, rpw, chunks):
+    # 3 interior strips + a ragged one; 101 rows end inside a chunk
+    got, ref = _run(gpu, variant, 3 * 128 + 44, 101, H, L, V, T, rpw, chunks)
+    assert mismatches(got, ref) == 0, f"{mismatches(got, ref)} points differ"
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 64), (256, 17), (1024, 256), (640, 333)])
+@pytest.mark.parametrize("variant", ["2DXYp", "2DXYnp"])
+def test_band_sizes(gpu, nx, ny, variant):
+    """Grid sizes where a band is one chunk, where every strip is interior,
+    and where the walker's one-wrap bound (rows >= V + rows per wave) decides
+    between the walker and the strip kernel (ny = 17 with 8 rows per wave)."""
+    for rpw, chunks in ((4, 2), (8, 1), (4, 16)):
+        got, ref = _run(gpu, variant, nx, ny, 5, 2, 5, 2, rpw, chunks)
+        assert mismatches(got, ref) == 0, (rpw, chunks)
+
+
+@pytest.mark.parametrize("variant", ["2DXYp", "2DXYnp", "2DYp", "2DXnp"])
+@pytest.mark.parametrize("nt", [1, 3])
+def test_band_host_streaming(gpu, variant, nt):
+    """Host-resident data streams through halo-extended staging tiles (rows
+    addressed linearly and clamped); the walker runs on each tile."""
+    torch = gpu
+    nx, ny = 520, 150
+    H, L, V, T = (5, 2, 5, 2)
+    if variant.startswith("2DX") and not variant.startswith("2DXY"):
+        V, T = 1, 0
+    if variant.startswith("2DY"):
+        H, L = 1, 0
+    u = synth.field(nx, ny, 301)
+    w = synth.uniform(H * V, 302, -1.0, 1.0)
+    init = np.full_like(u, SENTINEL)
+    hin = torch.from_numpy(u).pin_memory()
+    hout = torch.from_numpy(init).pin_memory()
+    h = api.cuSten_t()
+    create_weights(variant, h, nt, nx, ny, hout, hin, w, H, L, V, T)
+    api.custenSetBandConfig(h, 4, 2)
+    getattr(api, f"cuStenCompute{variant}")(h, api.HOST)
+    torch.cuda.synchronize()
+    getattr(api, f"cuStenDestroy{variant}")(h)
+    ref = oracle_weights(variant, u, w, H, L, V, T, init)
+    assert mismatches(hout.numpy(), ref) == 0
+
+
+def test_band_repeated_and_swapped(gpu):
+    """Several Computes with Swap in between: the ring and counters start
+    clean for every wave of every launch."""
+    torch = gpu
+    nx, ny = 768, 200
+    u = synth.field(nx, ny, 303)
+    w = synth.uniform(25, 304, -0.3, 0.3)
+    a, b, dw = dev(torch, u), dev(torch, np.zeros_like(u)), dev(torch, w)
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYp(h, 0, 1, nx, ny, 32, 32, b, a, dw, 5, 2, 2, 5, 2, 2)
+    api.custenSetBandConfig(h, 8, 3)
+    ref = u
+    for step in range(4):
+        api.cuStenCompute2DXYp(h, api.DEVICE)
+        api.cuStenSwap2DXYp(h, b if step % 2 == 0 else a)
+        ref = oracle.apply(ref, w, 5, 2, 5, 2, True)
+    torch.cuda.synchronize()
+    api.cuStenDestroy2DXYp(h)
+    assert bits_equal(a.cpu().numpy(), ref)
+
+
+def test_band_weighted_sum_function(gpu):
+    """The built-in weighted-sum point function runs on the stencil kernels,
+    so it takes the walker too."""
+    torch = gpu
+    nx, ny = 400, 130
+    u = synth.field(nx, ny, 305)
+    c = synth.uniform(25, 306, -1.0, 1.0)
+    init = np.full_like(u, SENTINEL)
+    du, dout, dc = dev(torch, u), dev(torch, init), dev(torch, c)
+    fn = api.custenGetBuiltinFunction("xy_weighted_sum")
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYpFun(h, 0, 1, nx, ny, 32, 8, dout, du, dc, 5, 2, 2, 5, 2, 2, fn)
+    api.custenSetBandConfig(h, 4, 4)
+    api.cuStenCompute2DXYpFun(h, api.DEVICE)
+    torch.cuda.synchronize()
+    api.cuStenDestroy2DXYpFun(h)
+    ref = oracle.apply_fun(oracle.FUN_XY_WSUM, 2, u, c, 5, 2, 5, 2, True, out=init.copy())
+    assert mismatches(dout.cpu().numpy(), ref) == 0
+
+
+def test_band_config_switch(gpu):
+    """custenSetTileConfig switches the walker off; both give the same bits."""
+    torch = gpu
+    nx, ny = 1024, 512
+    u = synth.field(nx, ny, 307)
+    w = synth.uniform(25, 308, -1.0, 1.0)
+    du, dw = dev(torch, u), dev(torch, w)
+    outs = []
+    for band in (True, False):
+        dout = dev(torch, np.zeros_like(u))
+        h = api.cuSten_t()
+        api.cuStenCreate2DXYp(h, 0, 1, nx, ny, 32, 32, dout, du, dw, 5, 2, 2, 5, 2, 2)
+        api.custenSetBandConfig(h, 4, 4)
+        if not band:
+            api.custenSetTileConfig(h, 4, 2)
+        assert api.custenGetBandMode(h) == (1 if band else 0)
+        api.cuStenCompute2DXYp(h, api.DEVICE)
+        torch.cuda.synchronize()
+        api.cuStenDestroy2DXYp(h)
+        outs.append(dout.cpu().numpy())
+    assert bits_equal(outs[0], outs[1])
+    assert bits_equal(outs[0], oracle.apply(u, w, 5, 2, 5, 2, True))

@@ -173,7 +173,7 @@ def test_autotuner_cache_file(gpu, tmp_path):
                         timeout=240)
     assert r1.returncode == 0, r1.stderr[-2000:]
     lines = cache.read_text().strip().splitlines()
-    assert len(lines) == 1 and len(lines[0].split()) == 11
+    assert len(lines) == 1 and len(lines[0].split()) == 12
     r2 = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                         timeout=240)
     assert r2.returncode == 0, r2.stderr[-2000:]
