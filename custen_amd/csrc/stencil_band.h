@@ -78,8 +78,10 @@ struct BandShape {
 // stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...).
 // Requirements (checked by the host): FP64, identity transform, the strip is
 // interior (full 128 columns, pair-aligned rows), and wrapped row indices
-// need at most one correction for every row the band touches.
-template <int H, int V, int RPW>
+// need at most one correction for every row the band touches.  FULL (also
+// decided by the host): every point of every walked strip is written and the
+// region height is a multiple of RPW, so each chunk stores RPW whole rows.
+template <int H, int V, int RPW, bool FULL>
 __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
 {
     using BS = BandShape<H, V, RPW>;
@@ -119,9 +121,15 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
         colR = colR >= nx ? nx - 1 : colR;
     }
 
-    auto loadRow = [&](int rel, int slot) {
-        const int sy = mapRowNear(a.rows, yS - a.T + rel);
-        const Glob<double>* row = in + (int64_t)sy * a.ldIn;
+    // Rows are loaded in order; syNext is the source row of the next one.
+    const RowMap rows = a.rows;
+    int gyNext = yS - a.T;
+    int syNext = mapRowNear(rows, gyNext);
+    auto loadRow = [&](int slot) {
+        const Glob<double>* row = in + (int64_t)syNext * a.ldIn;
+        gyNext++;
+        if (rows.wrap) syNext = syNext + 1 == rows.rowWrap ? 0 : syNext + 1;
+        else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
         double* l = ring + slot * LD;
         ldsDma16(row + x0 + 2 * lane, l + HLP);
         if constexpr (H > 1) {
@@ -131,12 +139,12 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
             }
         }
     };
-    auto loadChunk = [&](int rel0, int slot0) {
+    auto loadChunk = [&](int slot0) {
 #pragma unroll
         for (int r = 0; r < RPW; r++) {
             int sl = slot0 + r;
             if (sl >= S) sl -= S;
-            loadRow(rel0 + r, sl);
+            loadRow(sl);
         }
     };
 
@@ -145,11 +153,10 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     // (relative row) mod S.  Its loads are the rows [c*RPW + V-1, (c+1)*RPW + V-1);
     // chunk 0 also loads rows [0, V-1) first.
 #pragma unroll
-    for (int r = 0; r < V - 1; r++) loadRow(r, r);
-    int relLoad = V - 1, slotLoad = V - 1;
+    for (int r = 0; r < V - 1; r++) loadRow(r);
+    int slotLoad = V - 1;
     for (int c = 0; c < D; c++) {
-        if (c < nChunks) loadChunk(relLoad, slotLoad);
-        relLoad += RPW;
+        if (c < nChunks) loadChunk(slotLoad);
         slotLoad += RPW;
         if (slotLoad >= S) slotLoad -= S;
     }
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     int slotBase = 0;
     unsigned inexact = 0;  // bit i: chunk c-1-i did not store one row per instruction
     for (int c = 0; c < nChunks; c++) {
-        if (inexact & ((1u << D) - 1u)) {
+        if (!FULL && (inexact & ((1u << D) - 1u))) {
             waitVmcnt<0>();
         } else {
             const int la = min(c + D - 1, nChunks - 1) - c;
@@ -165,8 +172,7 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
             waitChunk<NL, RPW, D>(la, sa);
         }
         // chunk c + D reuses the slots of chunk c - 1's first RPW rows
-        if (c + D < nChunks) loadChunk(relLoad, slotLoad);
-        relLoad += RPW;
+        if (c + D < nChunks) loadChunk(slotLoad);
         slotLoad += RPW;
         if (slotLoad >= S) slotLoad -= S;
 
@@ -194,19 +200,29 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
             }
         }
 
-        bool exact = fullRow;
+        if constexpr (FULL) {
 #pragma unroll
-        for (int r = 0; r < RPW; r++) {
-            const int gy = yS + c * RPW + r;
-            const int gyg = a.yGlobal0 + gy;
-            if (gy >= yE || gyg < a.wy0 || gyg >= a.wy1) {
-                exact = false;
-                continue;
+            for (int r = 0; r < RPW; r++) {
+                const f64x2 o = {acc[r][0], acc[r][1]};
+                __builtin_nontemporal_store(
+                    o, reinterpret_cast<Glob<f64x2>*>(out + (int64_t)(yS + c * RPW + r) * a.ldOut +
+                                                      x0 + 2 * lane));
             }
-            storeLane<double, 2>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r], fullRow, nx,
-                                 a.wx0, a.wx1, a.zeroRight, R);
+        } else {
+            bool exact = fullRow;
+#pragma unroll
+            for (int r = 0; r < RPW; r++) {
+                const int gy = yS + c * RPW + r;
+                const int gyg = a.yGlobal0 + gy;
+                if (gy >= yE || gyg < a.wy0 || gyg >= a.wy1) {
+                    exact = false;
+                    continue;
+                }
+                storeLane<double, 2>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r], fullRow,
+                                     nx, a.wx0, a.wx1, a.zeroRight, R);
+            }
+            inexact = (inexact << 1) | (exact ? 0u : 1u);
         }
-        inexact = (inexact << 1) | (exact ? 0u : 1u);
         slotBase += RPW;
         if (slotBase >= S) slotBase -= S;
     }

@@ -367,8 +367,16 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
                 const int bandRows = RPW * s.chunks;
                 const int64_t waves = (int64_t)s.nStripTx * ((s.ny + bandRows - 1) / bandRows);
                 if (waves > 0x7fffffff) return hipErrorInvalidValue;
-                hipLaunchKernelGGL((dev::stencilBand<H, V, RPW>), dim3((unsigned)waves),
-                                   dim3(dev::kLanes), 0, st, s);
+                // every walked point written, whole chunks: no store masks
+                const bool full = lo * dev::kTW >= s.wx0 && hi * dev::kTW <= s.wx1 &&
+                                  s.yGlobal0 >= s.wy0 && s.yGlobal0 + s.ny <= s.wy1 &&
+                                  s.ny % RPW == 0;
+                if (full)
+                    hipLaunchKernelGGL((dev::stencilBand<H, V, RPW, true>), dim3((unsigned)waves),
+                                       dim3(dev::kLanes), 0, st, s);
+                else
+                    hipLaunchKernelGGL((dev::stencilBand<H, V, RPW, false>), dim3((unsigned)waves),
+                                       dim3(dev::kLanes), 0, st, s);
                 return hipGetLastError();
             }
         }

@@ -47,9 +47,12 @@ def _run(torch, variant, nx, ny, H, L, V, T, rpw, chunks, seed_extra=0):
 
 @pytest.mark.parametrize("variant,H,L,V,T", SHAPES)
 @pytest.mark.parametrize("rpw,chunks", [(4, 1), (4, 3), (8, 2)])
-def test_band_shapes(gpu, variant, H, L, V, T, rpw, chunks):
-    # 3 interior strips + a ragged one; 101 rows end inside a chunk
-    got, ref = _run(gpu, variant, 3 * 128 + 44, 101, H, L, V, T, rpw, chunks)
+@pytest.mark.parametrize("nx,ny", [(3 * 128 + 44, 101), (3 * 128, 96)])
+def test_band_shapes(gpu, variant, H, L, V, T, rpw, chunks, nx, ny):
+    # 428 x 101: 3 interior strips + a ragged one, 101 rows end inside a
+    # chunk (masked stores); 384 x 96: whole strips and chunks, which for the
+    # periodic variants is the unmasked (every point written) instantiation
+    got, ref = _run(gpu, variant, nx, ny, H, L, V, T, rpw, chunks)
     assert mismatches(got, ref) == 0, f"{mismatches(got, ref)} points differ"
 
 
@@ -65,12 +68,12 @@ def test_band_sizes(gpu, nx, ny, variant):
 
 
 @pytest.mark.parametrize("variant", ["2DXYp", "2DXYnp", "2DYp", "2DXnp"])
-@pytest.mark.parametrize("nt", [1, 3])
-def test_band_host_streaming(gpu, variant, nt):
+@pytest.mark.parametrize("nt,nx,ny", [(1, 520, 150), (3, 520, 150), (3, 512, 192)])
+def test_band_host_streaming(gpu, variant, nt, nx, ny):
     """Host-resident data streams through halo-extended staging tiles (rows
-    addressed linearly and clamped); the walker runs on each tile."""
+    addressed linearly and clamped); the walker runs on each tile (512 x 192
+    in 3 tiles: whole chunks, the unmasked instantiation when periodic)."""
     torch = gpu
-    nx, ny = 520, 150
     H, L, V, T = (5, 2, 5, 2)
     if variant.startswith("2DX") and not variant.startswith("2DXY"):
         V, T = 1, 0
