@@ -58,8 +58,8 @@ struct StencilLaunch {
     // tx = stripTx0 .. stripTx0 + nStripTx - 1, edge launches the listed tx
     int stripTx0, nStripTx;
     int edgeTx[4];
-    // 1: interior strips run the band walker (FP64, identity transform):
-    // one wave per rowsPerWave * chunks rows of a strip
+    // 1: interior strips run the band walker (identity transform): one wave
+    // per rowsPerWave * chunks rows of a strip
     int band;
 };
 
@@ -93,7 +93,7 @@ hipError_t builtinFunctionAddress(Builtin b, void** addr);
 // LDS bytes the stencil kernel uses for a given shape (reported in mem_shared).
 int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave);
 // LDS bytes of one band-walker workgroup (one wave).
-int stencilBandLdsBytes(int H, int V, int rowsPerWave);
+int stencilBandLdsBytes(Dtype dt, int H, int V, int rowsPerWave);
 
 // WENO5 upwind advection over a region of rows [0, ny), periodic in x: out =
 // u * Fx + v * Fy.  `in` rows are addressed through `rows` (the kernel reads

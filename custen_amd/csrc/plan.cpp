@@ -776,11 +776,9 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
                 std::vector<TuneVal> cands = {{4, 1, 2}, {4, 2, 2}, {4, 4, 2}, {8, 1, 2},
                                               {8, 2, 2}, {8, 4, 2}, {8, 8, 2}, {8, 16, 2}};
                 // band-walker candidates (rows per wave, chunks per band):
-                // FP64 weight stencils and weighted-sum point functions
-                const bool bandOk =
-                    p->dtype == Dtype::F64 &&
-                    (!p->fun || (p->dir == Dir::XY && !p->forceIndirect &&
-                                 p->builtin == Builtin::XYWeightedSum));
+                // weight stencils and the weighted-sum point function
+                const bool bandOk = !p->fun || (p->dir == Dir::XY && !p->forceIndirect &&
+                                                p->builtin == Builtin::XYWeightedSum);
                 if (bandOk) {
                     for (const TuneVal& b :
                          {TuneVal{4, 2, 2, 1}, TuneVal{4, 4, 2, 1}, TuneVal{4, 8, 2, 1},
@@ -1209,7 +1207,7 @@ void custenSetBandConfig(cuSten_t* h, int rowsPerWave, int chunksPerBand)
     p->chunks = chunksPerBand;
     p->colsPerLane = 2;
     p->band = true;
-    h->mem_shared = stencilBandLdsBytes(p->H, p->V, rowsPerWave);
+    h->mem_shared = stencilBandLdsBytes(p->dtype, p->H, p->V, rowsPerWave);
 }
 
 int custenGetBandMode(cuSten_t* h)

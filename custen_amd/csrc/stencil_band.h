@@ -1,10 +1,10 @@
 // stencil_band.h -- the band-walker stencil kernel: one wave streams a band
 // of rows down its own 128-column strip, with no workgroup barrier.
 //
-// Rows go from HBM straight into LDS (global_load_lds: each lane's 16 bytes
-// land at LDS base + 16 * lane, so one instruction stages a 128-column FP64
-// row body; two 4-byte loads by 2*HLP lanes stage the halo columns on each
-// side).  The wave keeps a ring of S row slots,
+// Rows go from HBM straight into LDS (global_load_lds: each lane's 16 or 4
+// bytes land at LDS base + 16 * lane or 4 * lane, so one instruction stages a
+// 128-column FP64 row body and two stage an FP32 one; one 4-byte load per
+// side by the lanes that cover HLP columns stages the halo).  The wave keeps a ring of S row slots,
 //     S = (D + 1) * RPW + V - 1,
 // and while it computes chunk c (RPW output rows from RPW + V - 1 input rows)
 // the rows of chunks c+1 .. c+D are in flight into the other slots, so no
@@ -58,39 +58,46 @@ __device__ __forceinline__ void ldsDma16(const Glob<double>* g, double* l)
 {
     __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 16, 0, 0);
 }
-__device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, double* l)
+template <typename T>
+__device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, T* l)
 {
     __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, 0);
 }
 
-// Ring geometry of a band-walker instantiation (FP64, 2 columns per lane).
-template <int H, int V, int RPW>
+// Ring geometry of a band-walker instantiation (2 columns per lane).  A row
+// body is one 16-byte load per lane for FP64 and two 4-byte loads (columns
+// lane and 64 + lane) for FP32; each halo side is one 4-byte load by the
+// lanes that cover HLP columns.
+template <typename T, int H, int V, int RPW>
 struct BandShape {
     static constexpr int D = 8 / RPW;                  // chunks in flight ahead
     static constexpr int HLP = ((H - 1) + 1) & ~1;     // halo columns staged per side
-    static constexpr int LD = kTW + 2 * HLP;           // slot length (doubles)
+    static constexpr int LD = kTW + 2 * HLP;           // slot length (elements)
     static constexpr int S = (D + 1) * RPW + V - 1;    // slots in the ring
-    static constexpr int NL = RPW * (H > 1 ? 3 : 1);   // load instructions per chunk
-    static constexpr int LDS_BYTES = S * LD * 8;
+    static constexpr int BODY = sizeof(T) == 8 ? 1 : 2;      // body loads per row
+    static constexpr int HALO_LANES = HLP * (int)sizeof(T) / 4;  // dwords per side
+    static constexpr int NL = RPW * (BODY + (H > 1 ? 2 : 0));  // load instructions per chunk
+    static constexpr int LDS_BYTES = S * LD * (int)sizeof(T);
 };
 
 // One wave per workgroup.  Wave g (after the XCD remap) walks interior strip
 // stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...).
-// Requirements (checked by the host): FP64, identity transform, the strip is
+// Requirements (checked by the host): identity transform, the strip is
 // interior (full 128 columns, pair-aligned rows), and wrapped row indices
 // need at most one correction for every row the band touches.  FULL (also
 // decided by the host): every point of every walked strip is written and the
 // region height is a multiple of RPW, so each chunk stores RPW whole rows.
-template <int H, int V, int RPW, bool FULL>
+template <typename T, int H, int V, int RPW, bool FULL>
 __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
 {
-    using BS = BandShape<H, V, RPW>;
+    using BS = BandShape<T, H, V, RPW>;
     constexpr int D = BS::D, HLP = BS::HLP, LD = BS::LD, S = BS::S, NL = BS::NL;
-    __shared__ __attribute__((aligned(16))) double ring[S * LD];
+    constexpr int HL = BS::HALO_LANES;
+    __shared__ __attribute__((aligned(16))) T ring[S * LD];
 
-    double w[H * V];
+    T w[H * V];
 #pragma unroll
-    for (int k = 0; k < H * V; k++) w[k] = a.weights[k];
+    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
 
     const int lane = threadIdx.x;
     const int g = xcdRemap(blockIdx.x, gridDim.x);
@@ -102,17 +109,19 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     const int yE = min(yS + bandRows, a.ny);
     const int nChunks = (yE - yS + RPW - 1) / RPW;
 
-    const Glob<double>* __restrict__ in = globalPtr<double>(a.in);
-    Glob<double>* __restrict__ out = globalPtr<double>(a.out);
+    const Glob<T>* __restrict__ in = globalPtr<T>(a.in);
+    Glob<T>* __restrict__ out = globalPtr<T>(a.out);
     const int nx = a.nx;
     const int x0 = tx * kTW;
     const int L = a.L;
     const int R = H - 1 - L;
     const bool fullRow = x0 >= a.wx0 && x0 + kTW <= a.wx1;
 
-    // halo columns staged for this lane (lanes < 2*HLP, one dword each)
-    int colL = x0 - HLP + (lane >> 1);
-    int colR = x0 + kTW + (lane >> 1);
+    // halo columns staged for this lane (lanes < HL, one dword each)
+    const int hcol = sizeof(T) == 8 ? lane >> 1 : lane;
+    const int hword = sizeof(T) == 8 ? lane & 1 : 0;
+    int colL = x0 - HLP + hcol;
+    int colR = x0 + kTW + hcol;
     if (a.xWrap) {
         if (colL < 0) colL += nx;
         if (colR >= nx) colR -= nx;
@@ -126,16 +135,21 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     int gyNext = yS - a.T;
     int syNext = mapRowNear(rows, gyNext);
     auto loadRow = [&](int slot) {
-        const Glob<double>* row = in + (int64_t)syNext * a.ldIn;
+        const Glob<T>* row = in + (int64_t)syNext * a.ldIn;
         gyNext++;
         if (rows.wrap) syNext = syNext + 1 == rows.rowWrap ? 0 : syNext + 1;
         else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
-        double* l = ring + slot * LD;
-        ldsDma16(row + x0 + 2 * lane, l + HLP);
+        T* l = ring + slot * LD;
+        if constexpr (sizeof(T) == 8) {
+            ldsDma16((const Glob<double>*)(row + x0 + 2 * lane), (double*)(l + HLP));
+        } else {
+            ldsDma4((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
+            ldsDma4((const Glob<unsigned>*)(row + x0 + kLanes + lane), l + HLP + kLanes);
+        }
         if constexpr (H > 1) {
-            if (lane < 2 * HLP) {
-                ldsDma4((const Glob<unsigned>*)(row + colL) + (lane & 1), l);
-                ldsDma4((const Glob<unsigned>*)(row + colR) + (lane & 1), l + HLP + kTW);
+            if (lane < HL) {
+                ldsDma4((const Glob<unsigned>*)(row + colL) + hword, l);
+                ldsDma4((const Glob<unsigned>*)(row + colR) + hword, l + HLP + kTW);
             }
         }
     };
@@ -176,15 +190,15 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
         slotLoad += RPW;
         if (slotLoad >= S) slotLoad -= S;
 
-        double acc[RPW][2];
+        T acc[RPW][2];
 #pragma unroll
-        for (int r = 0; r < RPW; r++) acc[r][0] = acc[r][1] = 0.0;
+        for (int r = 0; r < RPW; r++) acc[r][0] = acc[r][1] = T(0);
 #pragma unroll
         for (int ir = 0; ir < RPW + V - 1; ir++) {
             int sl = slotBase + ir;
             if (sl >= S) sl -= S;
-            const double* row = ring + sl * LD + (HLP - L) + 2 * lane;
-            double v[H + 1];
+            const T* row = ring + sl * LD + (HLP - L) + 2 * lane;
+            T v[H + 1];
 #pragma unroll
             for (int k = 0; k < H + 1; k++) v[k] = row[k];
 #pragma unroll
@@ -203,10 +217,11 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
         if constexpr (FULL) {
 #pragma unroll
             for (int r = 0; r < RPW; r++) {
-                const f64x2 o = {acc[r][0], acc[r][1]};
+                using V2 = typename Vec2<T>::type;
+                const V2 o = {acc[r][0], acc[r][1]};
                 __builtin_nontemporal_store(
-                    o, reinterpret_cast<Glob<f64x2>*>(out + (int64_t)(yS + c * RPW + r) * a.ldOut +
-                                                      x0 + 2 * lane));
+                    o, reinterpret_cast<Glob<V2>*>(out + (int64_t)(yS + c * RPW + r) * a.ldOut +
+                                                   x0 + 2 * lane));
             }
         } else {
             bool exact = fullRow;
@@ -218,8 +233,8 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
                     exact = false;
                     continue;
                 }
-                storeLane<double, 2>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r], fullRow,
-                                     nx, a.wx0, a.wx1, a.zeroRight, R);
+                storeLane<T, 2>(out + (int64_t)gy * a.ldOut, x0 + 2 * lane, acc[r], fullRow, nx,
+                                a.wx0, a.wx1, a.zeroRight, R);
             }
             inexact = (inexact << 1) | (exact ? 0u : 1u);
         }

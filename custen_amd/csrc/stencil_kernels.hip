@@ -14,7 +14,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
-#include <type_traits>
 
 #include "kernels.h"
 #include "stencil_band.h"
@@ -360,8 +359,7 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     if (hi > lo) {
         s.stripTx0 = lo;
         s.nStripTx = hi - lo;
-        if constexpr (std::is_same<T, double>::value && X == Xform::Identity && CPL == 2 &&
-                      (RPW == 4 || RPW == 8)) {
+        if constexpr (X == Xform::Identity && CPL == 2 && (RPW == 4 || RPW == 8)) {
             // band walker: every row it touches must need at most one wrap
             if (s.band && (!s.rows.wrap || (s.rows.rowWrap >= s.V + RPW && s.ny <= s.rows.rowWrap))) {
                 const int bandRows = RPW * s.chunks;
@@ -372,11 +370,11 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
                                   s.yGlobal0 >= s.wy0 && s.yGlobal0 + s.ny <= s.wy1 &&
                                   s.ny % RPW == 0;
                 if (full)
-                    hipLaunchKernelGGL((dev::stencilBand<H, V, RPW, true>), dim3((unsigned)waves),
-                                       dim3(dev::kLanes), 0, st, s);
+                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true>),
+                                       dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
                 else
-                    hipLaunchKernelGGL((dev::stencilBand<H, V, RPW, false>), dim3((unsigned)waves),
-                                       dim3(dev::kLanes), 0, st, s);
+                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false>),
+                                       dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
                 return hipGetLastError();
             }
         }
@@ -448,12 +446,12 @@ int stencilLdsBytes(Dtype dt, int H, int V, int rowsPerWave)
     return es * (dev::kWaves * rpw + V - 1) * dev::ldsRowLength(H);
 }
 
-int stencilBandLdsBytes(int H, int V, int rowsPerWave)
+int stencilBandLdsBytes(Dtype dt, int H, int V, int rowsPerWave)
 {
     const int rpw = rowsPerWave == 8 ? 8 : 4;
     const int d = 8 / rpw;
     const int hlp = H & ~1;  // H - 1 halo columns per side, rounded up to even
-    return 8 * ((d + 1) * rpw + V - 1) * (dev::kTW + 2 * hlp);
+    return (dt == Dtype::F64 ? 8 : 4) * ((d + 1) * rpw + V - 1) * (dev::kTW + 2 * hlp);
 }
 
 hipError_t launchStencil(const StencilLaunch& in, hipStream_t st)

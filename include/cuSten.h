@@ -305,10 +305,11 @@ void custenSetTileConfigEx(cuSten_t* pt_cuSten, int rowsPerWave, int chunksPerGr
 int custenGetTileConfig(cuSten_t* pt_cuSten, int* rowsPerWave, int* chunksPerGroup,
                         int* columnsPerLane);
 
-/* Band walker for FP64 weight stencils (and the weighted-sum point
- * function): each wave streams its own 128-column strip over a band of
- * rowsPerWave * chunksPerBand rows, with rows loaded from memory directly into
- * a ring of LDS rows, 8 rows per wave in flight ahead of the computed chunk.
+/* Band walker for weight stencils, FP64 and FP32 storage (and the
+ * weighted-sum point function): each wave streams its own 128-column strip
+ * over a band of rowsPerWave * chunksPerBand rows, with rows loaded from
+ * memory directly into a ring of LDS rows, 8 rows per wave in flight ahead of
+ * the computed chunk.
  * rowsPerWave 4 or 8, chunksPerBand >= 1.  Results are identical to the other
  * configurations; shapes or grids the walker does not cover use the strip
  * kernel with the same rowsPerWave and chunks.  custenSetTileConfig[Ex]

@@ -157,3 +157,35 @@ def test_band_config_switch(gpu):
         outs.append(dout.cpu().numpy())
     assert bits_equal(outs[0], outs[1])
     assert bits_equal(outs[0], oracle.apply(u, w, 5, 2, 5, 2, True))
+
+
+F32_SHAPES = [
+    # dir, H, L, V, T
+    (2, 5, 2, 5, 2), (2, 3, 1, 3, 1), (2, 7, 3, 7, 3), (0, 5, 2, 1, 0), (0, 11, 5, 1, 0),
+    (1, 1, 0, 5, 2), (1, 1, 0, 13, 6), (2, 3, 1, 5, 2),
+]
+
+
+@pytest.mark.parametrize("dir_,H,L,V,T", F32_SHAPES)
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("nx,ny", [(3 * 128 + 44, 101), (3 * 128, 96)])
+def test_band_f32(gpu, dir_, H, L, V, T, periodic, nx, ny):
+    """FP32 storage on the walker (each 128-column row body is two 4-byte
+    loads per lane): bit for bit against the FP32 oracle."""
+    torch = gpu
+    u = synth.field(nx, ny, 311 + H * 7 + V).astype(np.float32)
+    w64 = synth.uniform(H * V, 312, -1, 1)
+    init = np.full_like(u, np.float32(SENTINEL))
+    du, dout, dw = dev(torch, u), dev(torch, init), dev(torch, w64)
+    for rpw, chunks in ((4, 2), (8, 3)):
+        dout.copy_(dev(torch, init))
+        h = api.cuSten_t()
+        api.custenCreate2DF32(h, 0, 1, nx, ny, dir_, periodic, dout, du, dw, H, L, H - 1 - L, V,
+                              T, V - 1 - T)
+        api.custenSetBandConfig(h, rpw, chunks)
+        api.custenCompute2DF32(h, api.DEVICE)
+        torch.cuda.synchronize()
+        api.custenDestroy2DF32(h)
+        ref = oracle.apply(u, w64.astype(np.float32), H, L, V, T, bool(periodic),
+                           zero_right_edge=(dir_ == 0 and not periodic), out=init.copy())
+        assert mismatches(dout.cpu().numpy(), ref) == 0, (rpw, chunks)
