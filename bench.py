@@ -204,7 +204,7 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
         fn = compute_fn(api, kind, h)
         fn()
         torch.cuda.synchronize()
-        t = timer.run(fn, k2, 2)
+        t = timer.run(fn, k2, 20)
         extra[kind + "_8192"] = {"us": round(t * 1e3, 1), "GPts/s": round(pts / (t * 1e-3) / 1e9, 1),
                                  "GB/s": round(16.0 * pts / (t * 1e-3) / 1e9, 1)}
         if "fun" in kind:
@@ -221,7 +221,7 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
     fn = compute_fn(api, "xy_p_bih", h, f32=True)
     fn()
     torch.cuda.synchronize()
-    t = timer.run(fn, k2, 2)
+    t = timer.run(fn, k2, 20)
     destroy(api, "xy_p_bih", h, f32=True)
     got = dout32[:260].double().cpu().numpy()
     err_abs = float(np.max(np.abs(got[inner] - ref[inner])))
