@@ -164,6 +164,9 @@ _SIGS.update({
                                      c_int, c_int]),  # noqa: E501
     "custenBatchRun": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "custenBatchParity": (c_int, [c_void_p]),
+    "custenBatchSetTileConfig": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "custenBatchGetTileConfig": (c_int, [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                         ctypes.POINTER(c_int)]),
     "custenBatchDestroy": (None, [c_void_p]),
     "custenFillUniform": (None, [_P, c_size_t, c_uint64, c_double, c_double]),
     "custenCHCreate": (c_void_p, [c_int, ctypes.POINTER(custenCHParams), _P]),
@@ -585,6 +588,17 @@ class Batch:
     @property
     def parity(self) -> int:
         return lib().custenBatchParity(self.h)
+
+    def set_tile_config(self, rows_per_wave: int, chunks: int, band: bool = False) -> None:
+        if lib().custenBatchSetTileConfig(self.h, rows_per_wave, chunks, 1 if band else 0) != 0:
+            check("custenBatchSetTileConfig")
+
+    def tile_config(self):
+        r, c, b = c_int(), c_int(), c_int()
+        if lib().custenBatchGetTileConfig(self.h, ctypes.byref(r), ctypes.byref(c),
+                                          ctypes.byref(b)) != 0:
+            check("custenBatchGetTileConfig")
+        return r.value, c.value, b.value
 
     def close(self) -> None:
         if self.h:

@@ -2,9 +2,12 @@
 // together: one launch per step covers every strip of every grid (the
 // workgroup finds its grid by binary search over the strip prefix sums), and
 // a whole multi-step run is captured once into a HIP graph and replayed.
-// The per-grid launch loop (one launch per grid per step) is kept as the
-// baseline mode.  Per grid the arithmetic is the single-grid kernel's
-// (stencil_body.h), so results are bit-identical to cuStenCompute2DXYp.
+// With the band walker selected, a step is two launches: the band walker over
+// the interior 128-column strips of every grid (one wave per band, found the
+// same way), then the strip kernel over the remaining edge strips.  The
+// per-grid launch loop (one launch per grid per step) is kept as the baseline
+// mode.  Per grid the arithmetic is the single-grid kernels' (stencil_body.h,
+// stencil_band.h), so results are bit-identical to cuStenCompute2DXYp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +20,7 @@
 #include "../../include/custen_batch.h"
 #include "kernels.h"
 #include "plan_internal.h"
+#include "stencil_band.h"
 #include "stencil_body.h"
 #include "tuner.h"
 
@@ -29,38 +33,25 @@ struct BatchDev {
     void* out;
     int n;          // nx == ny == n for square grids; nx, ny kept separately below
     int nx, ny;
-    int stripBase;  // first strip id of this grid
+    int stripBase;  // first strip id of this grid (strip kernel over all strips)
     int vec;        // rows pair-aligned
+    // band-walker split: strips [0, nBandTx) are walked by bands, strips
+    // [nBandTx, tilesX) run the strip kernel
+    int bandBase;   // first band item (wave) of this grid
+    int nBandTx;
+    int edgeBase;   // first edge-strip item of this grid
 };
 
 namespace dev {
 
-__device__ __forceinline__ int findGrid(const BatchDev* g, int count, int item)
+// The single-grid launch description of grid G (periodic xy, all points
+// written), reading the side given by the step parity.
+__device__ __forceinline__ StencilLaunch gridLaunch(const BatchDev& G, int parity, Dtype dt,
+                                                    const double* weights, int H, int L, int V,
+                                                    int Tt, int rpw, int chunks)
 {
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (g[mid].stripBase <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-template <typename T, int H, int V, int RPW>
-__global__ __launch_bounds__(kThreads) void batchKernel(const BatchDev* __restrict__ grids,
-                                                        int count, int firstItem, int parity,
-                                                        const double* __restrict__ weights, int L,
-                                                        int Tt, int chunks)
-{
-    __shared__ __attribute__((aligned(16))) T tile[(kWaves * RPW + V - 1) * ldsRowLength(H)];
-    T w[H * V];
-#pragma unroll
-    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(weights[k]);
-    const int item = firstItem + xcdRemap(blockIdx.x, gridDim.x);
-    const int gi = findGrid(grids, count, item);
-    const BatchDev& G = grids[gi];
     StencilLaunch a;
-    a.dtype = sizeof(T) == 8 ? Dtype::F64 : Dtype::F32;
+    a.dtype = dt;
     a.in = parity ? G.out : G.in;
     a.out = parity ? const_cast<void*>(G.in) : G.out;
     a.weights = weights;
@@ -84,11 +75,63 @@ __global__ __launch_bounds__(kThreads) void batchKernel(const BatchDev* __restri
     a.yGlobal0 = 0;
     a.zeroRight = 0;
     a.xform = Xform::Identity;
-    a.rowsPerWave = RPW;
+    a.rowsPerWave = rpw;
     a.chunks = chunks;
     a.vecIn = G.vec;
     a.vecOut = G.vec;
-    stencilStrip<T, H, V, RPW, Xform::Identity>(a, w, tile, item - G.stripBase);
+    a.band = 0;
+    a.stripTx0 = 0;
+    a.nStripTx = 0;
+    return a;
+}
+
+// Strip kernel over every strip of every grid (edges = 0) or over the edge
+// strips [nBandTx, tilesX) of every grid (edges = 1).
+// itemGrid[item] is the grid that owns the item (built on the host with the
+// item tables, so a workgroup finds its grid with one load).
+template <typename T, int H, int V, int RPW>
+__global__ __launch_bounds__(kThreads) void batchKernel(const BatchDev* __restrict__ grids,
+                                                        const int* __restrict__ itemGrid,
+                                                        int firstItem, int parity,
+                                                        const double* __restrict__ weights, int L,
+                                                        int Tt, int chunks, int edges)
+{
+    __shared__ __attribute__((aligned(16))) T tile[(kWaves * RPW + V - 1) * ldsRowLength(H)];
+    T w[H * V];
+#pragma unroll
+    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(weights[k]);
+    const int item = firstItem + xcdRemap(blockIdx.x, gridDim.x);
+    const BatchDev& G = grids[itemGrid[item]];
+    const StencilLaunch a = gridLaunch(G, parity, sizeof(T) == 8 ? Dtype::F64 : Dtype::F32,
+                                       weights, H, L, V, Tt, RPW, chunks);
+    const int tilesX = (G.nx + kTW - 1) / kTW;
+    const int tx0 = edges ? G.nBandTx : 0;
+    const int k = item - (edges ? G.edgeBase : G.stripBase);
+    const int tx = tx0 + k % (tilesX - tx0), tyg = k / (tilesX - tx0);
+    if (stripIsInterior(a, tx)) stencilStripImpl<T, H, V, RPW, Xform::Identity, true>(a, w, tile, tx, tyg);
+    else stencilStripImpl<T, H, V, RPW, Xform::Identity, false>(a, w, tile, tx, tyg);
+}
+
+// Band walker over the interior strips [0, nBandTx) of every grid.
+template <typename T, int H, int V, int RPW>
+__global__ __launch_bounds__(kLanes) void batchBandKernel(const BatchDev* __restrict__ grids,
+                                                          const int* __restrict__ itemGrid,
+                                                          int parity,
+                                                          const double* __restrict__ weights,
+                                                          int L, int Tt, int chunks)
+{
+    using BS = BandShape<T, H, V, RPW>;
+    __shared__ __attribute__((aligned(16))) T ring[BS::S * BS::LD];
+    T w[H * V];
+#pragma unroll
+    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(weights[k]);
+    const int item = xcdRemap(blockIdx.x, gridDim.x);
+    const BatchDev& G = grids[itemGrid[item]];
+    StencilLaunch a = gridLaunch(G, parity, sizeof(T) == 8 ? Dtype::F64 : Dtype::F32, weights, H,
+                                 L, V, Tt, RPW, chunks);
+    a.nStripTx = G.nBandTx;
+    if (G.ny % RPW == 0) bandWalk<T, H, V, RPW, true>(a, w, ring, item - G.bandBase);
+    else bandWalk<T, H, V, RPW, false>(a, w, ring, item - G.bandBase);
 }
 
 }  // namespace dev
@@ -102,10 +145,15 @@ struct custenBatch {
     Dtype dtype = Dtype::F64;
     int H = 5, L = 2, V = 5, T = 2;
     int rpw = 8, chunks = 4;
+    bool band = false;        // fused steps use the band walker for interior strips
     std::vector<BatchDev> host;
     std::vector<int> strips;  // strips per grid
     int totalStrips = 0;
+    int totalBand = 0, totalEdge = 0;  // band items and edge-strip items (band mode)
     BatchDev* dGrids = nullptr;
+    // owning grid of every item: all strips, band items, edge-strip items
+    int* dItemGrid[3] = {nullptr, nullptr, nullptr};
+    size_t itemCap[3] = {0, 0, 0};
     double* dWeights = nullptr;
     int parity = 0;           // 1: the current input is the original output side
     bool tuned = false;
@@ -124,22 +172,31 @@ int stripsFor(int nx, int ny, int rpw, int chunks)
 }
 
 template <typename T, int H, int V, int RPW>
-hipError_t launchOne(const custenBatch* b, int count, const BatchDev* grids, int firstItem,
-                     int nItems, int parity, hipStream_t st)
+hipError_t launchOne(const custenBatch* b, const BatchDev* grids, int firstItem, int nItems,
+                     int parity, int kind, hipStream_t st)
 {
-    hipLaunchKernelGGL((dev::batchKernel<T, H, V, RPW>), dim3(nItems), dim3(dev::kThreads), 0, st,
-                       grids, count, firstItem, parity, b->dWeights, b->L, b->T, b->chunks);
+    if (kind == 2)
+        hipLaunchKernelGGL((dev::batchBandKernel<T, H, V, RPW>), dim3(nItems), dim3(dev::kLanes),
+                           0, st, grids, b->dItemGrid[2], parity, b->dWeights, b->L, b->T,
+                           b->chunks);
+    else
+        hipLaunchKernelGGL((dev::batchKernel<T, H, V, RPW>), dim3(nItems), dim3(dev::kThreads), 0,
+                           st, grids, b->dItemGrid[kind], firstItem, parity, b->dWeights, b->L,
+                           b->T, b->chunks, kind);
     return hipGetLastError();
 }
 
-hipError_t launchItems(const custenBatch* b, int firstItem, int nItems, int parity, hipStream_t st)
+// kind 0: strip kernel over items [firstItem, +nItems) of the all-strips
+// table; 1: strip kernel over the edge-strip table; 2: band walker over the
+// band table (firstItem must be 0 for kinds 1 and 2).
+hipError_t launchItems(const custenBatch* b, int firstItem, int nItems, int parity, hipStream_t st,
+                       int kind = 0)
 {
     if (nItems <= 0) return hipSuccess;
-    const int count = b->numGrids;
     const BatchDev* g = b->dGrids;
 #define CUSTEN_BATCH_CASE(TT, HH, VV, RR)                                              \
     if (b->H == HH && b->V == VV && b->rpw == RR)                                      \
-        return launchOne<TT, HH, VV, RR>(b, count, g, firstItem, nItems, parity, st);
+        return launchOne<TT, HH, VV, RR>(b, g, firstItem, nItems, parity, kind, st);
     if (b->dtype == Dtype::F64) {
         CUSTEN_BATCH_CASE(double, 5, 5, 8)
         CUSTEN_BATCH_CASE(double, 5, 5, 4)
@@ -155,28 +212,85 @@ hipError_t launchItems(const custenBatch* b, int firstItem, int nItems, int pari
     return hipErrorInvalidValue;
 }
 
-// Strip tables for a tile configuration (host copy and device copy).
-hipError_t configure(custenBatch* b, int rpw, int chunks)
+// Item tables for a tile configuration (host copy and device copy): all
+// strips (per-grid loop and strip-only fused steps) and, with the band
+// walker, its band items and the remaining edge strips.  A grid's strips are
+// walked by bands when its rows are pair-aligned and its height allows one
+// wrap correction per row (ny >= V + rows per wave).
+hipError_t configure(custenBatch* b, int rpw, int chunks, bool band)
 {
     b->rpw = rpw;
     b->chunks = chunks;
-    int64_t strips = 0;
+    b->band = band;
+    int64_t strips = 0, bands = 0, edges = 0;
     for (int g = 0; g < b->numGrids; g++) {
-        b->host[g].stripBase = (int)strips;
-        b->strips[g] = stripsFor(b->host[g].nx, b->host[g].ny, rpw, chunks);
+        BatchDev& d = b->host[g];
+        d.stripBase = (int)strips;
+        b->strips[g] = stripsFor(d.nx, d.ny, rpw, chunks);
         strips += b->strips[g];
+        const int tilesX = (d.nx + dev::kTW - 1) / dev::kTW;
+        d.nBandTx = (band && d.vec && d.ny >= b->V + rpw) ? d.nx / dev::kTW : 0;
+        d.bandBase = (int)bands;
+        d.edgeBase = (int)edges;
+        const int bandRows = rpw * chunks;
+        const int groupRows = dev::kWaves * rpw * chunks;
+        bands += (int64_t)d.nBandTx * ((d.ny + bandRows - 1) / bandRows);
+        edges += (int64_t)(tilesX - d.nBandTx) * ((d.ny + groupRows - 1) / groupRows);
     }
-    if (strips > (int64_t)0x7fffffff) return hipErrorInvalidValue;
+    if (strips > (int64_t)0x7fffffff || bands > (int64_t)0x7fffffff ||
+        edges > (int64_t)0x7fffffff)
+        return hipErrorInvalidValue;
     b->totalStrips = (int)strips;
+    b->totalBand = (int)bands;
+    b->totalEdge = (int)edges;
+    // owning grid of every item of the three tables
+    std::vector<int> owner[3];
+    owner[0].reserve(b->totalStrips);
+    owner[1].reserve(b->totalEdge);
+    owner[2].reserve(b->totalBand);
+    for (int g = 0; g < b->numGrids; g++) {
+        const BatchDev& d = b->host[g];
+        const int nBand = (g + 1 < b->numGrids ? b->host[g + 1].bandBase : b->totalBand) - d.bandBase;
+        const int nEdge = (g + 1 < b->numGrids ? b->host[g + 1].edgeBase : b->totalEdge) - d.edgeBase;
+        owner[0].insert(owner[0].end(), b->strips[g], g);
+        owner[1].insert(owner[1].end(), nEdge, g);
+        owner[2].insert(owner[2].end(), nBand, g);
+    }
+    for (int k = 0; k < 3; k++) {
+        const size_t n = std::max<size_t>(1, owner[k].size());
+        if (n > b->itemCap[k]) {
+            if (b->dItemGrid[k]) (void)hipFree(b->dItemGrid[k]);
+            b->dItemGrid[k] = nullptr;
+            b->itemCap[k] = 0;
+            hipError_t e = hipMalloc(&b->dItemGrid[k], n * sizeof(int));
+            if (e != hipSuccess) return e;
+            b->itemCap[k] = n;
+        }
+        if (!owner[k].empty()) {
+            hipError_t e = hipMemcpy(b->dItemGrid[k], owner[k].data(), owner[k].size() * sizeof(int),
+                                     hipMemcpyHostToDevice);
+            if (e != hipSuccess) return e;
+        }
+    }
     return hipMemcpy(b->dGrids, b->host.data(), sizeof(BatchDev) * b->numGrids,
                      hipMemcpyHostToDevice);
 }
 
-// `steps` applies starting at parity p0, all grids in one launch per step.
+// One fused step at parity p: one launch over all strips, or the band walker
+// then the edge strips.
+hipError_t enqueueStep(const custenBatch* b, int p, hipStream_t st)
+{
+    if (!b->band) return launchItems(b, 0, b->totalStrips, p, st, 0);
+    hipError_t e = launchItems(b, 0, b->totalBand, p, st, 2);
+    if (e == hipSuccess) e = launchItems(b, 0, b->totalEdge, p, st, 1);
+    return e;
+}
+
+// `steps` applies starting at parity p0.
 hipError_t enqueueFused(const custenBatch* b, int steps, int p0, hipStream_t st)
 {
     for (int s = 0; s < steps; s++) {
-        hipError_t e = launchItems(b, 0, b->totalStrips, (p0 + s) & 1, st);
+        hipError_t e = enqueueStep(b, (p0 + s) & 1, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -249,8 +363,7 @@ extern "C" custenBatch* custenBatchCreate(int deviceNum, int numGrids, const int
     }
     b->totalStrips = (int)strips;
     if ((e = hipMalloc(&b->dGrids, sizeof(BatchDev) * numGrids)) != hipSuccess ||
-        (e = hipMemcpy(b->dGrids, b->host.data(), sizeof(BatchDev) * numGrids,
-                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = configure(b, b->rpw, b->chunks, false)) != hipSuccess ||
         (e = hipMalloc(&b->dWeights, sizeof(double) * numStenHoriz * numStenVert)) != hipSuccess ||
         (e = hipMemcpy(b->dWeights, weights, sizeof(double) * numStenHoriz * numStenVert,
                        hipMemcpyDefault)) != hipSuccess ||
@@ -299,22 +412,25 @@ extern "C" int custenBatchRun(custenBatch* b, int steps, int mode, hipStream_t s
             k.count = b->numGrids;
             TuneVal v;
             if (!tunerLookup(k, v)) {
-                static const std::vector<TuneVal> cands = {{4, 1}, {4, 4}, {8, 1}, {8, 2},
-                                                           {8, 4}, {8, 8}};
+                // strip kernel (rows per wave, chunks per workgroup), then
+                // band walker (rows per wave, chunks per band)
+                static const std::vector<TuneVal> cands = {
+                    {4, 1}, {4, 4}, {8, 1}, {8, 2}, {8, 4}, {8, 8},
+                    {4, 2, 2, 1}, {4, 4, 2, 1}, {8, 1, 2, 1}, {8, 2, 2, 1}};
                 auto run = [&](const TuneVal& c) {
                     hipError_t err = hipSuccess;
-                    if (c.rpw != b->rpw || c.chunks != b->chunks) {
+                    if (c.rpw != b->rpw || c.chunks != b->chunks || (c.band != 0) != b->band) {
                         err = hipStreamSynchronize(b->own);
-                        if (err == hipSuccess) err = configure(b, c.rpw, c.chunks);
+                        if (err == hipSuccess) err = configure(b, c.rpw, c.chunks, c.band != 0);
                     }
                     if (err != hipSuccess) return err;
-                    return launchItems(b, 0, b->totalStrips, b->parity, b->own);
+                    return enqueueStep(b, b->parity, b->own);
                 };
                 e = tunerMeasure(b->own, run, cands, 2, v);
                 if (e == hipSuccess) tunerStore(k, v);
             }
             if (e == hipSuccess) e = hipStreamSynchronize(b->own);
-            if (e == hipSuccess) e = configure(b, v.rpw, v.chunks);
+            if (e == hipSuccess) e = configure(b, v.rpw, v.chunks, v.band != 0);
             if (e != hipSuccess) {
                 setHipError(e, fn);
                 return lastError();
@@ -354,6 +470,47 @@ extern "C" int custenBatchRun(custenBatch* b, int steps, int mode, hipStream_t s
 
 extern "C" int custenBatchParity(const custenBatch* b) { return b ? b->parity : -1; }
 
+extern "C" int custenBatchSetTileConfig(custenBatch* b, int rowsPerWave, int chunks, int band)
+{
+    const char* fn = "custenBatchSetTileConfig";
+    if (!b || (rowsPerWave != 4 && rowsPerWave != 8) || chunks < 1 || (band != 0 && band != 1)) {
+        setError(CUSTEN_ERROR_INVALID_VALUE,
+                 std::string(fn) + ": rowsPerWave must be 4 or 8, chunks >= 1, band 0 or 1");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    (void)hipSetDevice(b->device);
+    // the device tables are rewritten: no earlier step may still read them,
+    // and graphs captured for the old item counts are discarded
+    hipError_t e = hipStreamSynchronize(b->own);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) {
+        for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto& kv : b->graphDefs) (void)hipGraphDestroy(kv.second);
+        b->graphs.clear();
+        b->graphDefs.clear();
+        e = configure(b, rowsPerWave, chunks, band != 0);
+    }
+    if (e != hipSuccess) {
+        setHipError(e, fn);
+        return lastError();
+    }
+    b->tuned = true;
+    return CUSTEN_SUCCESS;
+}
+
+extern "C" int custenBatchGetTileConfig(const custenBatch* b, int* rowsPerWave, int* chunks,
+                                        int* band)
+{
+    if (!b) {
+        setError(CUSTEN_ERROR_INVALID_VALUE, "custenBatchGetTileConfig: null batch");
+        return CUSTEN_ERROR_INVALID_VALUE;
+    }
+    if (rowsPerWave) *rowsPerWave = b->tuned ? b->rpw : 0;
+    if (chunks) *chunks = b->tuned ? b->chunks : 0;
+    if (band) *band = b->tuned && b->band ? 1 : 0;
+    return CUSTEN_SUCCESS;
+}
+
 extern "C" void custenBatchDestroy(custenBatch* b)
 {
     if (!b) return;
@@ -363,6 +520,8 @@ extern "C" void custenBatchDestroy(custenBatch* b)
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& kv : b->graphDefs) (void)hipGraphDestroy(kv.second);
     if (b->dGrids) (void)hipFree(b->dGrids);
+    for (int k = 0; k < 3; k++)
+        if (b->dItemGrid[k]) (void)hipFree(b->dItemGrid[k]);
     if (b->dWeights) (void)hipFree(b->dWeights);
     if (b->own) (void)hipStreamDestroy(b->own);
     delete b;

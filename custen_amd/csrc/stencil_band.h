@@ -80,27 +80,24 @@ struct BandShape {
     static constexpr int LDS_BYTES = S * LD * (int)sizeof(T);
 };
 
-// One wave per workgroup.  Wave g (after the XCD remap) walks interior strip
-// stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...).
-// Requirements (checked by the host): identity transform, the strip is
-// interior (full 128 columns, pair-aligned rows), and wrapped row indices
-// need at most one correction for every row the band touches.  FULL (also
-// decided by the host): every point of every walked strip is written and the
-// region height is a multiple of RPW, so each chunk stores RPW whole rows.
+// Band walk of wave g of a launch (one wave per workgroup): interior strip
+// stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...) with
+// band = g / nStripTx.  `ring` is the wave's BandShape::S x LD LDS ring and
+// `w` the H*V weights.  Requirements (checked by the host): identity
+// transform, the strip is interior (full 128 columns, pair-aligned rows), and
+// wrapped row indices need at most one correction for every row the band
+// touches.  FULL (also decided by the host): every point of every walked
+// strip is written and the region height is a multiple of RPW, so each chunk
+// stores RPW whole rows.
 template <typename T, int H, int V, int RPW, bool FULL>
-__global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
+__device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H * V],
+                                         T* __restrict__ ring, int g)
 {
     using BS = BandShape<T, H, V, RPW>;
     constexpr int D = BS::D, HLP = BS::HLP, LD = BS::LD, S = BS::S, NL = BS::NL;
     constexpr int HL = BS::HALO_LANES;
-    __shared__ __attribute__((aligned(16))) T ring[S * LD];
-
-    T w[H * V];
-#pragma unroll
-    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
 
     const int lane = threadIdx.x;
-    const int g = xcdRemap(blockIdx.x, gridDim.x);
     const int tx = a.stripTx0 + g % a.nStripTx;
     const int band = g / a.nStripTx;
     const int bandRows = RPW * a.chunks;
@@ -241,6 +238,18 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
         slotBase += RPW;
         if (slotBase >= S) slotBase -= S;
     }
+}
+
+// Single-region launch: one wave per workgroup, waves in XCD-aware order.
+template <typename T, int H, int V, int RPW, bool FULL>
+__global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
+{
+    using BS = BandShape<T, H, V, RPW>;
+    __shared__ __attribute__((aligned(16))) T ring[BS::S * BS::LD];
+    T w[H * V];
+#pragma unroll
+    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
+    bandWalk<T, H, V, RPW, FULL>(a, w, ring, xcdRemap(blockIdx.x, gridDim.x));
 }
 
 }  // namespace dev

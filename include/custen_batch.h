@@ -29,6 +29,16 @@ int custenBatchRun(custenBatch* batch, int steps, int mode, hipStream_t stream);
 /* Where the current field of every grid is: 0 = dataInput[g] (before the
  * first step and after an even number of steps), 1 = dataOutput[g]. */
 int custenBatchParity(const custenBatch* batch);
+/* Tile configuration of the fused steps (results do not depend on it):
+ * band = 0: strip kernel, rowsPerWave 4 or 8 and chunks >= 1 row chunks per
+ * workgroup; band = 1: band walker for the interior 128-column strips of
+ * every grid (chunks per band), strip kernel for the rest.  Replaces the
+ * autotuner's choice; graphs captured before are discarded.  Waits for the
+ * batch's earlier work on its own stream. */
+int custenBatchSetTileConfig(custenBatch* batch, int rowsPerWave, int chunks, int band);
+/* The configuration in use: *band is 1 for the band walker.  All zero before
+ * the first fused run when neither the autotuner nor the caller chose one. */
+int custenBatchGetTileConfig(const custenBatch* batch, int* rowsPerWave, int* chunks, int* band);
 void custenBatchDestroy(custenBatch* batch);
 
 #ifdef __cplusplus

@@ -123,6 +123,9 @@ def test_batch_validation_without_gpu():
     assert not L.custenBatchCreate(0, 1, nx, nx, ptr, ptr, w, 7, 3, 7, 3, 0)
     assert L.custenGetLastError() == 4  # shape not supported for batches
     api.clear_error()
+    assert L.custenBatchSetTileConfig(None, 4, 2, 1) == 1
+    assert L.custenBatchGetTileConfig(None, None, None, None) == 1
+    api.clear_error()
 
 
 def test_ch_restore_rejects_bad_files(tmp_path):

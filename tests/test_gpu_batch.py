@@ -11,8 +11,14 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("f32", [False, True])
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batch_matches_oracle(gpu, f32, mode):
+@pytest.mark.parametrize("mode,config", [(0, None), (1, None), (0, (8, 4, False)),
+                                         (0, (4, 2, True)), (0, (8, 1, True)),
+                                         (0, (4, 3, True)), (1, (4, 2, True))])
+def test_batch_matches_oracle(gpu, f32, mode, config):
+    """config None: the autotuner's choice; otherwise (rows per wave, chunks,
+    band walker).  Grids narrower than 128 columns, of 128 and 256 columns and
+    of 200 (one walked strip and one edge strip) exercise both kernels of a
+    band-walker step."""
     torch = gpu
     sizes = [64, 128, 40, 256, 128, 96, 200]
     steps = 3
@@ -26,6 +32,9 @@ def test_batch_matches_oracle(gpu, f32, mode):
     outs = [torch.zeros_like(t) for t in ins]
     dw = torch.from_numpy(w).cuda()
     b = api.Batch(outs, ins, dw, f32=f32)
+    if config is not None:
+        b.set_tile_config(*config)
+        assert b.tile_config() == (config[0], config[1], int(config[2]))
     b.run(steps, mode=mode)
     torch.cuda.synchronize()
     assert b.parity == steps % 2
