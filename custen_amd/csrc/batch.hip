@@ -130,8 +130,9 @@ __global__ __launch_bounds__(kLanes) void batchBandKernel(const BatchDev* __rest
     StencilLaunch a = gridLaunch(G, parity, sizeof(T) == 8 ? Dtype::F64 : Dtype::F32, weights, H,
                                  L, V, Tt, RPW, chunks);
     a.nStripTx = G.nBandTx;
-    if (G.ny % RPW == 0) bandWalk<T, H, V, RPW, true>(a, w, ring, item - G.bandBase);
-    else bandWalk<T, H, V, RPW, false>(a, w, ring, item - G.bandBase);
+    Lds<T>* r = (Lds<T>*)ring;
+    if (G.ny % RPW == 0) bandWalk<T, H, V, RPW, true>(a, w, r, item - G.bandBase);
+    else bandWalk<T, H, V, RPW, false>(a, w, r, item - G.bandBase);
 }
 
 }  // namespace dev

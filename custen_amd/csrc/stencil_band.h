@@ -4,7 +4,8 @@
 // Rows go from HBM straight into LDS (global_load_lds: each lane's 16 or 4
 // bytes land at LDS base + 16 * lane or 4 * lane, so one instruction stages a
 // 128-column FP64 row body and two stage an FP32 one; one 4-byte load per
-// side by the lanes that cover HLP columns stages the halo).  The wave keeps a ring of S row slots,
+// side by the lanes that cover HLP columns stages the halo).  The wave keeps
+// a ring of S row slots,
 //     S = (D + 1) * RPW + V - 1,
 // and while it computes chunk c (RPW output rows from RPW + V - 1 input rows)
 // the rows of chunks c+1 .. c+D are in flight into the other slots, so no
@@ -27,7 +28,11 @@
 namespace custen {
 namespace dev {
 
+// LDS (address space 3) views: the ring is addressed through them from the
+// kernel down, so the compiler never has to turn a generic pointer into an
+// LDS address.
 typedef __attribute__((address_space(3))) void LdsVoid;
+template <typename T> using Lds = __attribute__((address_space(3))) T;
 
 // s_waitcnt with vmcnt = N and the other counters unconstrained (gfx9
 // encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
@@ -54,12 +59,12 @@ __device__ __forceinline__ void waitChunk(int la, int sa)
     }
 }
 
-__device__ __forceinline__ void ldsDma16(const Glob<double>* g, double* l)
+__device__ __forceinline__ void ldsDma16(const Glob<double>* g, Lds<double>* l)
 {
     __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 16, 0, 0);
 }
 template <typename T>
-__device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, T* l)
+__device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, Lds<T>* l)
 {
     __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, 0);
 }
@@ -91,7 +96,7 @@ struct BandShape {
 // stores RPW whole rows.
 template <typename T, int H, int V, int RPW, bool FULL>
 __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H * V],
-                                         T* __restrict__ ring, int g)
+                                         Lds<T>* __restrict__ ring, int g)
 {
     using BS = BandShape<T, H, V, RPW>;
     constexpr int D = BS::D, HLP = BS::HLP, LD = BS::LD, S = BS::S, NL = BS::NL;
@@ -136,9 +141,9 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
         gyNext++;
         if (rows.wrap) syNext = syNext + 1 == rows.rowWrap ? 0 : syNext + 1;
         else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
-        T* l = ring + slot * LD;
+        Lds<T>* l = ring + slot * LD;
         if constexpr (sizeof(T) == 8) {
-            ldsDma16((const Glob<double>*)(row + x0 + 2 * lane), (double*)(l + HLP));
+            ldsDma16((const Glob<double>*)(row + x0 + 2 * lane), (Lds<double>*)(l + HLP));
         } else {
             ldsDma4((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
             ldsDma4((const Glob<unsigned>*)(row + x0 + kLanes + lane), l + HLP + kLanes);
@@ -194,10 +199,23 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
         for (int ir = 0; ir < RPW + V - 1; ir++) {
             int sl = slotBase + ir;
             if (sl >= S) sl -= S;
-            const T* row = ring + sl * LD + (HLP - L) + 2 * lane;
+            const Lds<T>* row = ring + sl * LD + (HLP - L) + 2 * lane;
             T v[H + 1];
+            if (((HLP - L) & 1) == 0 && (H + 1) % 2 == 0) {
+                // window pair-aligned: one 2-element read per pair (FP64:
+                // ds_read_b128, 16 lanes x 16 B cover all banks once)
+                using V2 = typename Vec2<T>::type;
+                const Lds<V2>* rp = (const Lds<V2>*)row;
 #pragma unroll
-            for (int k = 0; k < H + 1; k++) v[k] = row[k];
+                for (int k = 0; k < (H + 1) / 2; k++) {
+                    const V2 p = rp[k];
+                    v[2 * k] = p.x;
+                    v[2 * k + 1] = p.y;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < H + 1; k++) v[k] = row[k];
+            }
 #pragma unroll
             for (int r = 0; r < RPW; r++) {
                 const int j = ir - r;
@@ -249,7 +267,7 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     T w[H * V];
 #pragma unroll
     for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
-    bandWalk<T, H, V, RPW, FULL>(a, w, ring, xcdRemap(blockIdx.x, gridDim.x));
+    bandWalk<T, H, V, RPW, FULL>(a, w, (Lds<T>*)ring, xcdRemap(blockIdx.x, gridDim.x));
 }
 
 }  // namespace dev

@@ -26,6 +26,9 @@ SHAPES = [
     ("2DXYp", 3, 1, 3, 1), ("2DXYnp", 3, 1, 3, 1), ("2DXYp", 5, 2, 5, 2), ("2DXYnp", 5, 2, 5, 2),
     ("2DXYp", 5, 0, 5, 4), ("2DXYp", 3, 1, 5, 2), ("2DXYp", 5, 2, 3, 1), ("2DXYp", 7, 3, 7, 3),
     ("2DXYnp", 9, 4, 9, 4),
+    # one-sided windows: all of the halo on one side of the strip or band
+    ("2DXYp", 5, 4, 5, 0), ("2DXYnp", 3, 2, 5, 0), ("2DXp", 7, 6, 1, 0), ("2DXnp", 9, 0, 1, 0),
+    ("2DYp", 1, 0, 7, 0), ("2DYnp", 1, 0, 9, 8),
 ]
 
 
