@@ -47,8 +47,16 @@ class Timer:
         self.torch = torch
         self.stream = stream
 
-    def run(self, fn, steps, warmup):
+    def run(self, fn, steps, warmup, prewarm_s=0.0):
+        """ms per step of `steps` calls of fn, after `warmup` untimed calls.
+        prewarm_s: first call fn for about that many seconds (untimed), so
+        that the GPU clocks are back up after a host-side pause."""
         torch = self.torch
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < prewarm_s:
+            for _ in range(20):
+                fn()
+            torch.cuda.synchronize()
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -176,7 +184,7 @@ def bench_stencil_8192(args, torch, api, extra):
     log(f"xy_p_bih 8192^2: {ms * 1e3:.1f} us/apply")
     # measured copy ceiling: device-to-device copy of the same 512 MiB (also the
     # calibration dispatch for the rocprof traffic counters)
-    tc = timer.run(lambda: dout.copy_(din), max(5, args.steps // 4), 2)
+    tc = timer.run(lambda: dout.copy_(din), max(5, args.steps // 4), 2, prewarm_s=0.05)
     extra["copy_ceiling_GBps"] = round(2 * 8.0 * pts / (tc * 1e-3) / 1e9, 1)
     if not args.no_extra:
         bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref)
@@ -204,7 +212,7 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
         fn = compute_fn(api, kind, h)
         fn()
         torch.cuda.synchronize()
-        t = timer.run(fn, k2, 20)
+        t = timer.run(fn, k2, 2, prewarm_s=0.05)
         extra[kind + "_8192"] = {"us": round(t * 1e3, 1), "GPts/s": round(pts / (t * 1e-3) / 1e9, 1),
                                  "GB/s": round(16.0 * pts / (t * 1e-3) / 1e9, 1)}
         if "fun" in kind:
@@ -221,7 +229,7 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
     fn = compute_fn(api, "xy_p_bih", h, f32=True)
     fn()
     torch.cuda.synchronize()
-    t = timer.run(fn, k2, 20)
+    t = timer.run(fn, k2, 2, prewarm_s=0.05)
     destroy(api, "xy_p_bih", h, f32=True)
     got = dout32[:260].double().cpu().numpy()
     err_abs = float(np.max(np.abs(got[inner] - ref[inner])))
@@ -265,7 +273,7 @@ def bench_weno_8192(torch, api, stream, timer, din, dout, steps, extra):
             fn = lambda: api.cuStenCompute2DXYWENOADVp(h, api.DEVICE)  # noqa: E731
             fn()
             torch.cuda.synchronize()
-            t = timer.run(fn, steps, 2)
+            t = timer.run(fn, steps, 2, prewarm_s=0.05)
             api.cuStenDestroy2DXYWENOADVp(h)
             res[f"{'single' if single else 'double'}_squares_rpw{rpw}"] = {
                 "us": round(t * 1e3, 1), "GPts/s": round(pts / (t * 1e-3) / 1e9, 1),
@@ -326,7 +334,7 @@ def bench_pent(args, torch, api, extra):
     b = rhs.clone()
     sol = lambda: api.custenPentSolveBatch(*work, b, m, batch,  # noqa: E731
                                            stream=stream.cuda_stream)
-    sol_ms = Timer(torch, stream).run(sol, reps, 1)
+    sol_ms = Timer(torch, stream).run(sol, reps, 1, prewarm_s=0.05)
     # parity on the first 256 systems: factor again from pristine, one solve
     for w, p in zip(work, pristine):
         w.copy_(p)
@@ -369,7 +377,8 @@ def bench_x_1024(args, torch, api, extra):
     api.custenSetStream(h, stream.cuda_stream)
     api.cuStenCompute2DXp(h, api.DEVICE)
     torch.cuda.synchronize()
-    t = Timer(torch, stream).run(lambda: api.cuStenCompute2DXp(h, api.DEVICE), 200, 10)
+    t = Timer(torch, stream).run(lambda: api.cuStenCompute2DXp(h, api.DEVICE), 200, 10,
+                                 prewarm_s=0.05)
     res = {"us": round(t * 1e3, 2), "GPts/s": round(n * n / (t * 1e-3) / 1e9, 1),
            "effective_GB/s": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}
     # the same 100 applies captured once in a HIP graph and replayed: the
