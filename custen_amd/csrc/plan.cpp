@@ -776,9 +776,12 @@ void computeCommon(cuSten_t* h, bool offload, const char* fn)
                 std::vector<TuneVal> cands = {{4, 1, 2}, {4, 2, 2}, {4, 4, 2}, {8, 1, 2},
                                               {8, 2, 2}, {8, 4, 2}, {8, 8, 2}, {8, 16, 2}};
                 // band-walker candidates (rows per wave, chunks per band):
-                // weight stencils and the weighted-sum point function
-                const bool bandOk = !p->fun || (p->dir == Dir::XY && !p->forceIndirect &&
-                                                p->builtin == Builtin::XYWeightedSum);
+                // weight stencils and the weighted-sum and cube-minus point
+                // functions (both run on the stencil kernels)
+                const bool bandOk =
+                    !p->fun || (p->dir == Dir::XY && !p->forceIndirect &&
+                                (p->builtin == Builtin::XYWeightedSum ||
+                                 p->builtin == Builtin::XYCubeMinus));
                 if (bandOk) {
                     for (const TuneVal& b :
                          {TuneVal{4, 2, 2, 1}, TuneVal{4, 4, 2, 1}, TuneVal{4, 8, 2, 1},

@@ -19,6 +19,9 @@
 //
 // Arithmetic is the fma chain of the other stencil kernels (row j outer,
 // column i inner, from 0.0), so every configuration gives identical bits.
+// The point-wise transform of the cube-minus point function (u^3 - u,
+// unfused) is applied to each value as it is read from the ring; the strip
+// kernel applies the same function while staging, so the values agree.
 #pragma once
 
 #include "stencil_tile.h"
@@ -88,13 +91,13 @@ struct BandShape {
 // Band walk of wave g of a launch (one wave per workgroup): interior strip
 // stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...) with
 // band = g / nStripTx.  `ring` is the wave's BandShape::S x LD LDS ring and
-// `w` the H*V weights.  Requirements (checked by the host): identity
-// transform, the strip is interior (full 128 columns, pair-aligned rows), and
-// wrapped row indices need at most one correction for every row the band
-// touches.  FULL (also decided by the host): every point of every walked
-// strip is written and the region height is a multiple of RPW, so each chunk
-// stores RPW whole rows.
-template <typename T, int H, int V, int RPW, bool FULL>
+// `w` the H*V weights; X the point-wise transform of the values read.
+// Requirements (checked by the host): the strip is interior (full 128
+// columns, pair-aligned rows), and wrapped row indices need at most one
+// correction for every row the band touches.  FULL (also decided by the
+// host): every point of every walked strip is written and the region height
+// is a multiple of RPW, so each chunk stores RPW whole rows.
+template <typename T, int H, int V, int RPW, bool FULL, Xform X = Xform::Identity>
 __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H * V],
                                          Lds<T>* __restrict__ ring, int g)
 {
@@ -209,12 +212,12 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
 #pragma unroll
                 for (int k = 0; k < (H + 1) / 2; k++) {
                     const V2 p = rp[k];
-                    v[2 * k] = p.x;
-                    v[2 * k + 1] = p.y;
+                    v[2 * k] = xformValue<T, X>(p.x);
+                    v[2 * k + 1] = xformValue<T, X>(p.y);
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < H + 1; k++) v[k] = row[k];
+                for (int k = 0; k < H + 1; k++) v[k] = xformValue<T, X>(row[k]);
             }
 #pragma unroll
             for (int r = 0; r < RPW; r++) {
@@ -259,7 +262,7 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
 }
 
 // Single-region launch: one wave per workgroup, waves in XCD-aware order.
-template <typename T, int H, int V, int RPW, bool FULL>
+template <typename T, int H, int V, int RPW, bool FULL, Xform X = Xform::Identity>
 __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
 {
     using BS = BandShape<T, H, V, RPW>;
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     T w[H * V];
 #pragma unroll
     for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
-    bandWalk<T, H, V, RPW, FULL>(a, w, (Lds<T>*)ring, xcdRemap(blockIdx.x, gridDim.x));
+    bandWalk<T, H, V, RPW, FULL, X>(a, w, (Lds<T>*)ring, xcdRemap(blockIdx.x, gridDim.x));
 }
 
 }  // namespace dev

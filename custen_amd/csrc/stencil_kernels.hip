@@ -359,7 +359,12 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     if (hi > lo) {
         s.stripTx0 = lo;
         s.nStripTx = hi - lo;
-        if constexpr (X == Xform::Identity && CPL == 2 && (RPW == 4 || RPW == 8)) {
+        // band walker: every weight stencil shape, and the cube-minus point
+        // function on its xy shapes 3x3 and 5x5
+        constexpr bool bandShape =
+            X == Xform::Identity ||
+            (X == Xform::CubeMinus && H == V && (H == 3 || H == 5) && sizeof(T) == 8);
+        if constexpr (bandShape && CPL == 2 && (RPW == 4 || RPW == 8)) {
             // band walker: every row it touches must need at most one wrap
             if (s.band && (!s.rows.wrap || (s.rows.rowWrap >= s.V + RPW && s.ny <= s.rows.rowWrap))) {
                 const int bandRows = RPW * s.chunks;
@@ -370,10 +375,10 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
                                   s.yGlobal0 >= s.wy0 && s.yGlobal0 + s.ny <= s.wy1 &&
                                   s.ny % RPW == 0;
                 if (full)
-                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true>),
+                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true, X>),
                                        dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
                 else
-                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false>),
+                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false, X>),
                                        dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
                 return hipGetLastError();
             }

@@ -306,7 +306,8 @@ int custenGetTileConfig(cuSten_t* pt_cuSten, int* rowsPerWave, int* chunksPerGro
                         int* columnsPerLane);
 
 /* Band walker for weight stencils, FP64 and FP32 storage (and the
- * weighted-sum point function): each wave streams its own 128-column strip
+ * weighted-sum and cube-minus point functions, xy 3x3 and 5x5 for the
+ * latter): each wave streams its own 128-column strip
  * over a band of rowsPerWave * chunksPerBand rows, with rows loaded from
  * memory directly into a ring of LDS rows, 8 rows per wave in flight ahead of
  * the computed chunk.

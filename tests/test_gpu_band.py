@@ -192,3 +192,30 @@ def test_band_f32(gpu, dir_, H, L, V, T, periodic, nx, ny):
         ref = oracle.apply(u, w64.astype(np.float32), H, L, V, T, bool(periodic),
                            zero_right_edge=(dir_ == 0 and not periodic), out=init.copy())
         assert mismatches(dout.cpu().numpy(), ref) == 0, (rpw, chunks)
+
+
+@pytest.mark.parametrize("variant", ["2DXYpFun", "2DXYnpFun"])
+@pytest.mark.parametrize("H,L,V,T", [(3, 1, 3, 1), (5, 2, 5, 2), (3, 0, 3, 2)])
+@pytest.mark.parametrize("nx,ny", [(3 * 128 + 44, 101), (3 * 128, 96)])
+def test_band_cube_minus_function(gpu, variant, H, L, V, T, nx, ny):
+    """The cube-minus point function (u^3 - u under a weighted sum, as the
+    Cahn-Hilliard non-linear term) on the walker: the transform is applied as
+    values are read from the ring, and the result equals the oracle's
+    restatement of the function bit for bit."""
+    torch = gpu
+    u = synth.field(nx, ny, 321 + H)
+    c = synth.uniform(H * V, 322, -1.0, 1.0)
+    init = np.full_like(u, SENTINEL)
+    fn = api.custenGetBuiltinFunction("xy_cube_minus")
+    for rpw, chunks in ((4, 2), (8, 3)):
+        du, dout, dc = dev(torch, u), dev(torch, init), dev(torch, c)
+        h = api.cuSten_t()
+        getattr(api, f"cuStenCreate{variant}")(h, 0, 1, nx, ny, 32, 8, dout, du, dc, H, L,
+                                               H - 1 - L, V, T, V - 1 - T, fn)
+        api.custenSetBandConfig(h, rpw, chunks)
+        getattr(api, f"cuStenCompute{variant}")(h, api.DEVICE)
+        torch.cuda.synchronize()
+        getattr(api, f"cuStenDestroy{variant}")(h)
+        ref = oracle.apply_fun(oracle.FUN_XY_CUBE_MINUS, 2, u, c, H, L, V, T,
+                               variant == "2DXYpFun", out=init.copy())
+        assert mismatches(dout.cpu().numpy(), ref) == 0, (rpw, chunks)
