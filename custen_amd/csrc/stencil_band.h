@@ -62,14 +62,31 @@ __device__ __forceinline__ void waitChunk(int la, int sa)
     }
 }
 
+// Cache-policy operand of the direct-to-LDS loads: 2 = non-temporal, 0 =
+// default.  Row bodies of FP64 stencils at most 5 rows tall are loaded
+// non-temporally: at 8192^2 that measured 1.5 % faster for 5x5, 4 % for 3x3
+// and 6 % for x 11-point.  Taller stencils (whose V-1 band-halo rows are read
+// again by the neighbouring band) and FP32 fields (a 256 MiB field partly
+// stays in the last-level cache between applies) measured 10-40 % slower
+// with it, so they keep the default, as do the halo columns, which
+// neighbouring strips read again.
+constexpr int kDmaStream = 2;
+constexpr int kDmaDefault = 0;
+template <typename T, int V>
+constexpr int bodyPolicy()
+{
+    return (sizeof(T) == 8 && V <= 5) ? kDmaStream : kDmaDefault;
+}
+
+template <int POLICY>
 __device__ __forceinline__ void ldsDma16(const Glob<double>* g, Lds<double>* l)
 {
-    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 16, 0, POLICY);
 }
-template <typename T>
+template <int POLICY, typename T>
 __device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, Lds<T>* l)
 {
-    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, POLICY);
 }
 
 // Ring geometry of a band-walker instantiation (2 columns per lane).  A row
@@ -146,15 +163,17 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
         else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
         Lds<T>* l = ring + slot * LD;
         if constexpr (sizeof(T) == 8) {
-            ldsDma16((const Glob<double>*)(row + x0 + 2 * lane), (Lds<double>*)(l + HLP));
+            ldsDma16<bodyPolicy<T, V>()>((const Glob<double>*)(row + x0 + 2 * lane),
+                                         (Lds<double>*)(l + HLP));
         } else {
-            ldsDma4((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
-            ldsDma4((const Glob<unsigned>*)(row + x0 + kLanes + lane), l + HLP + kLanes);
+            ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
+            ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + kLanes + lane),
+                                        l + HLP + kLanes);
         }
         if constexpr (H > 1) {
             if (lane < HL) {
-                ldsDma4((const Glob<unsigned>*)(row + colL) + hword, l);
-                ldsDma4((const Glob<unsigned>*)(row + colR) + hword, l + HLP + kTW);
+                ldsDma4<kDmaDefault>((const Glob<unsigned>*)(row + colL) + hword, l);
+                ldsDma4<kDmaDefault>((const Glob<unsigned>*)(row + colR) + hword, l + HLP + kTW);
             }
         }
     };
@@ -204,7 +223,10 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
             if (sl >= S) sl -= S;
             const Lds<T>* row = ring + sl * LD + (HLP - L) + 2 * lane;
             T v[H + 1];
-            if (((HLP - L) & 1) == 0 && (H + 1) % 2 == 0) {
+            // Pair reads only for stencils of at most 25 points: the wider
+            // ones hold their weights in spilled scalar registers and run
+            // slower with a second read path in the loop.
+            if (H * V <= 25 && ((HLP - L) & 1) == 0 && (H + 1) % 2 == 0) {
                 // window pair-aligned: one 2-element read per pair (FP64:
                 // ds_read_b128, 16 lanes x 16 B cover all banks once)
                 using V2 = typename Vec2<T>::type;
