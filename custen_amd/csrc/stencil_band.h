@@ -64,8 +64,9 @@ __device__ __forceinline__ void waitChunk(int la, int sa)
 
 // Cache-policy operand of the direct-to-LDS loads: 2 = non-temporal, 0 =
 // default.  Row bodies of FP64 stencils at most 5 rows tall are loaded
-// non-temporally: at 8192^2 that measured 1.5 % faster for 5x5, 4 % for 3x3
-// and 6 % for x 11-point.  Taller stencils (whose V-1 band-halo rows are read
+// non-temporally, except the rows that a neighbouring band reads too: at
+// 8192^2 that measured 1.5 % faster for 5x5, 4 % for 3x3 and 6 % for x
+// 11-point.  Taller stencils (whose V-1 band-halo rows are read
 // again by the neighbouring band) and FP32 fields (a 256 MiB field partly
 // stays in the last-level cache between applies) measured 10-40 % slower
 // with it, so they keep the default, as do the halo columns, which
@@ -156,15 +157,24 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
     const RowMap rows = a.rows;
     int gyNext = yS - a.T;
     int syNext = mapRowNear(rows, gyNext);
+    // The first and last V-1 rows of a band's window are also read by the
+    // bands above and below: they keep the default cache policy so that the
+    // second read finds them in L2.
+    const int gy0 = yS - a.T;
     auto loadRow = [&](int slot) {
         const Glob<T>* row = in + (int64_t)syNext * a.ldIn;
+        const int rel = gyNext - gy0;
+        const bool shared = rel < V - 1 || rel >= bandRows;
         gyNext++;
         if (rows.wrap) syNext = syNext + 1 == rows.rowWrap ? 0 : syNext + 1;
         else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
         Lds<T>* l = ring + slot * LD;
         if constexpr (sizeof(T) == 8) {
-            ldsDma16<bodyPolicy<T, V>()>((const Glob<double>*)(row + x0 + 2 * lane),
-                                         (Lds<double>*)(l + HLP));
+            const Glob<double>* g = (const Glob<double>*)(row + x0 + 2 * lane);
+            if (bodyPolicy<T, V>() == kDmaStream && !shared)
+                ldsDma16<kDmaStream>(g, (Lds<double>*)(l + HLP));
+            else
+                ldsDma16<kDmaDefault>(g, (Lds<double>*)(l + HLP));
         } else {
             ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
             ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + kLanes + lane),
