@@ -62,22 +62,14 @@ __device__ __forceinline__ void waitChunk(int la, int sa)
     }
 }
 
-// Cache-policy operand of the direct-to-LDS loads: 2 = non-temporal, 0 =
-// default.  Row bodies of FP64 stencils at most 5 rows tall are loaded
-// non-temporally, except the rows that a neighbouring band reads too: at
-// 8192^2 that measured 1.5 % faster for 5x5, 4 % for 3x3 and 6 % for x
-// 11-point.  Taller stencils (whose V-1 band-halo rows are read
-// again by the neighbouring band) and FP32 fields (a 256 MiB field partly
-// stays in the last-level cache between applies) measured 10-40 % slower
-// with it, so they keep the default, as do the halo columns, which
-// neighbouring strips read again.
-constexpr int kDmaStream = 2;
+// Direct-to-LDS loads use the default cache policy.  Non-temporal row loads
+// were measured: 6-10 % faster when the same input is applied again and
+// again (the unchanged input then stays partly in the last-level cache), but
+// within 2 % either way in a time-stepping loop that alternates input and
+// output, and 10-40 % slower for tall stencils and FP32 fields.  Since the
+// autotuner times repeated applies of one input, that policy would also
+// mislead it, so it is not used.
 constexpr int kDmaDefault = 0;
-template <typename T, int V>
-constexpr int bodyPolicy()
-{
-    return (sizeof(T) == 8 && V <= 5) ? kDmaStream : kDmaDefault;
-}
 
 template <int POLICY>
 __device__ __forceinline__ void ldsDma16(const Glob<double>* g, Lds<double>* l)
@@ -157,28 +149,19 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
     const RowMap rows = a.rows;
     int gyNext = yS - a.T;
     int syNext = mapRowNear(rows, gyNext);
-    // The first and last V-1 rows of a band's window are also read by the
-    // bands above and below: they keep the default cache policy so that the
-    // second read finds them in L2.
-    const int gy0 = yS - a.T;
     auto loadRow = [&](int slot) {
         const Glob<T>* row = in + (int64_t)syNext * a.ldIn;
-        const int rel = gyNext - gy0;
-        const bool shared = rel < V - 1 || rel >= bandRows;
         gyNext++;
         if (rows.wrap) syNext = syNext + 1 == rows.rowWrap ? 0 : syNext + 1;
         else syNext = min(max(gyNext, rows.rowMin), rows.rowMax);
         Lds<T>* l = ring + slot * LD;
         if constexpr (sizeof(T) == 8) {
-            const Glob<double>* g = (const Glob<double>*)(row + x0 + 2 * lane);
-            if (bodyPolicy<T, V>() == kDmaStream && !shared)
-                ldsDma16<kDmaStream>(g, (Lds<double>*)(l + HLP));
-            else
-                ldsDma16<kDmaDefault>(g, (Lds<double>*)(l + HLP));
+            ldsDma16<kDmaDefault>((const Glob<double>*)(row + x0 + 2 * lane),
+                                  (Lds<double>*)(l + HLP));
         } else {
-            ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
-            ldsDma4<bodyPolicy<T, V>()>((const Glob<unsigned>*)(row + x0 + kLanes + lane),
-                                        l + HLP + kLanes);
+            ldsDma4<kDmaDefault>((const Glob<unsigned>*)(row + x0 + lane), l + HLP);
+            ldsDma4<kDmaDefault>((const Glob<unsigned>*)(row + x0 + kLanes + lane),
+                                 l + HLP + kLanes);
         }
         if constexpr (H > 1) {
             if (lane < HL) {
