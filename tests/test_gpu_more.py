@@ -181,10 +181,12 @@ def test_autotuner_cache_file(gpu, tmp_path):
     assert len(cache.read_text().strip().splitlines()) == 1  # reused, not re-tuned
 
 
-def test_compute_captured_in_a_graph(gpu):
+@pytest.mark.parametrize("band", [False, True])
+def test_compute_captured_in_a_graph(gpu, band):
     """Compute launched into a caller's HIP graph (via custenSetStream on the
     capturing stream): every replay, with eager work in between that changes
-    the input, matches the oracle on the input of that moment."""
+    the input, matches the oracle on the input of that moment (strip kernel
+    and band walker)."""
     torch = gpu
     n = 512
     u0 = synth.field(n, n, seed=71)
@@ -194,6 +196,8 @@ def test_compute_captured_in_a_graph(gpu):
     dw = torch.from_numpy(w).cuda()
     h = api.cuSten_t()
     api.cuStenCreate2DXYp(h, 0, 1, n, n, 32, 32, dout, din, dw, 5, 2, 2, 5, 2, 2)
+    if band:
+        api.custenSetBandConfig(h, 4, 2)
     s = torch.cuda.Stream()
     api.custenSetStream(h, s.cuda_stream)
     with torch.cuda.stream(s):
