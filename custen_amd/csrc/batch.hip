@@ -82,6 +82,7 @@ __device__ __forceinline__ StencilLaunch gridLaunch(const BatchDev& G, int parit
     a.band = 0;
     a.stripTx0 = 0;
     a.nStripTx = 0;
+    a.edgeList = 0;
     return a;
 }
 

@@ -54,9 +54,10 @@ struct StencilLaunch {
     int chunks;              // row chunks per workgroup, 0 = default
     int colsPerLane;         // 1 or 2 (0 = default 2)
     int vecIn, vecOut;       // set by launchStencil: rows are pair-aligned
-    // column strips handled by one launch: interior launches cover
-    // tx = stripTx0 .. stripTx0 + nStripTx - 1, edge launches the listed tx
+    // column strips handled by one launch: tx = stripTx0 .. stripTx0 +
+    // nStripTx - 1, or (edgeList = 1) the nStripTx listed in edgeTx
     int stripTx0, nStripTx;
+    int edgeList;
     int edgeTx[4];
     // 1: interior strips run the band walker (identity transform): one wave
     // per rowsPerWave * chunks rows of a strip

@@ -126,13 +126,13 @@ __host__ __device__ inline bool stripIsInterior(const StencilLaunch& a, int tx, 
            (a.rows.wrap ? a.rows.rowWrap >= a.V : true);
 }
 
-// Strip g of a single-grid launch -> (tx, tyg).
-__device__ __forceinline__ void stripOf(const StencilLaunch& a, bool interior, int g, int& tx,
-                                        int& tyg)
+// Strip g of a single-grid launch -> (tx, tyg): the listed edge strips, or
+// the contiguous range stripTx0 .. stripTx0 + nStripTx - 1.
+__device__ __forceinline__ void stripOf(const StencilLaunch& a, int g, int& tx, int& tyg)
 {
     const int k = g % a.nStripTx;
     tyg = g / a.nStripTx;
-    tx = interior ? a.stripTx0 + k : a.edgeTx[k];
+    tx = a.edgeList ? a.edgeTx[k] : a.stripTx0 + k;
 }
 
 // Batched grids: one strip id covers all columns of a grid; both paths live in

@@ -52,7 +52,7 @@ __attribute__((amdgpu_waves_per_eu(FixedOccupancy<T, H, V, RPW, CPL>::value, 8))
 #pragma unroll
     for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
     int tx, tyg;
-    stripOf(a, FAST, xcdRemap(blockIdx.x, gridDim.x), tx, tyg);
+    stripOf(a, xcdRemap(blockIdx.x, gridDim.x), tx, tyg);
     stencilStripImpl<T, H, V, RPW, X, FAST, CPL>(a, w, tile, tx, tyg);
 }
 
@@ -324,8 +324,10 @@ int gridSize(int nx, int ny, int TH, int chunks)
     return tilesX * tilesY;
 }
 
-// Interior column strips run the FAST kernel, the (at most four) edge strips
-// the general one.
+// Interior column strips run the FAST kernel (or the band walker), the edge
+// strips the general one: by list when there are at most four, otherwise (no
+// interior strip at all: unaligned rows, or a periodic grid lower than the
+// stencil) as the contiguous ranges left and right of the interior.
 template <typename T, int H, int V, int RPW, Xform X, int CPL = 2>
 hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
 {
@@ -339,24 +341,41 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
     while (hi > lo && !dev::stripIsInterior(s, hi - 1, CPL)) hi--;
     for (int t = lo; t < hi; t++)
         if (!dev::stripIsInterior(s, t, CPL)) return hipErrorInvalidValue;  // not contiguous
-    StencilLaunch e = s;
-    e.nStripTx = 0;
-    for (int t = 0; t < tilesX; t++) {
-        if (t >= lo && t < hi) continue;
-        if (e.nStripTx == 4) return hipErrorInvalidValue;
-        e.edgeTx[e.nStripTx++] = t;
-    }
     // edge strips first, one row chunk per workgroup: they are a few percent
     // of the work and must not leave a tail after the interior launch
-    if (e.nStripTx > 0) {
-        e.chunks = 1;
-        const int tilesYe = (s.ny + dev::kWaves * RPW - 1) / (dev::kWaves * RPW);
+    const int tilesYe = (s.ny + dev::kWaves * RPW - 1) / (dev::kWaves * RPW);
+    auto launchEdges = [&](const StencilLaunch& e) -> hipError_t {
+        const int64_t groups = (int64_t)e.nStripTx * tilesYe;
+        if (groups > 0x7fffffff) return hipErrorInvalidValue;
         hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, false, CPL>),
-                           dim3(e.nStripTx * tilesYe), dim3(dev::kThreads), 0, st, e);
-        hipError_t err = hipGetLastError();
+                           dim3((unsigned)groups), dim3(dev::kThreads), 0, st, e);
+        return hipGetLastError();
+    };
+    const int nEdge = tilesX - (hi - lo);
+    if (nEdge > 0 && nEdge <= 4) {
+        StencilLaunch e = s;
+        e.chunks = 1;
+        e.edgeList = 1;
+        e.nStripTx = 0;
+        for (int t = 0; t < tilesX; t++)
+            if (t < lo || t >= hi) e.edgeTx[e.nStripTx++] = t;
+        hipError_t err = launchEdges(e);
         if (err != hipSuccess) return err;
+    } else if (nEdge > 4) {
+        const int ranges[2][2] = {{0, lo}, {hi, tilesX}};
+        for (const auto& r : ranges) {
+            if (r[1] <= r[0]) continue;
+            StencilLaunch e = s;
+            e.chunks = 1;
+            e.edgeList = 0;
+            e.stripTx0 = r[0];
+            e.nStripTx = r[1] - r[0];
+            hipError_t err = launchEdges(e);
+            if (err != hipSuccess) return err;
+        }
     }
     if (hi > lo) {
+        s.edgeList = 0;
         s.stripTx0 = lo;
         s.nStripTx = hi - lo;
         // band walker: every weight stencil shape, and the cube-minus point

@@ -219,3 +219,27 @@ def test_band_cube_minus_function(gpu, variant, H, L, V, T, nx, ny):
         ref = oracle.apply_fun(oracle.FUN_XY_CUBE_MINUS, 2, u, c, H, L, V, T,
                                variant == "2DXYpFun", out=init.copy())
         assert mismatches(dout.cpu().numpy(), ref) == 0, (rpw, chunks)
+
+
+def _random_cases(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        variant, H, L, V, T = SHAPES[int(rng.integers(len(SHAPES)))]
+        # mostly even widths (pair-aligned rows: the walker runs), some odd
+        # (unaligned rows: every strip falls back to the strip kernel)
+        nx = int(rng.integers(64, 750)) * 2 if rng.random() < 0.75 else int(rng.integers(128, 1500))
+        ny = int(rng.integers(V + 8, 400))
+        rpw = int(rng.choice([4, 8]))
+        chunks = int(rng.integers(1, 6))
+        out.append((variant, nx, ny, H, L, V, T, rpw, chunks))
+    return out
+
+
+@pytest.mark.parametrize("variant,nx,ny,H,L,V,T,rpw,chunks", _random_cases(24, 2024))
+def test_band_random_geometry(gpu, variant, nx, ny, H, L, V, T, rpw, chunks):
+    """Seeded random grid sizes, shapes and band configurations (widths that
+    are and are not multiples of 128, heights that do and do not fill whole
+    chunks and bands), bit for bit against the oracle."""
+    got, ref = _run(gpu, variant, nx, ny, H, L, V, T, rpw, chunks, seed_extra=nx * ny)
+    assert mismatches(got, ref) == 0, f"{mismatches(got, ref)} points differ"

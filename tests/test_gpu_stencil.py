@@ -78,6 +78,14 @@ CASES = [
     ("2DXYnp", 128, 64, 2, 5, 2, 5, 2),
     ("2DXYnp", 257, 131, 1, 3, 1, 3, 1),
     ("2DXYnp", 150, 80, 4, 4, 1, 3, 2),
+    # more than four strips and none of them interior (odd widths: rows not
+    # pair-aligned; a periodic grid lower than the stencil)
+    ("2DYnp", 973, 123, 1, 1, 0, 5, 2),
+    ("2DXYp", 1001, 64, 1, 5, 2, 5, 2),
+    ("2DXp", 999, 40, 1, 5, 2, 1, 0),
+    ("2DXYnp", 777, 50, 1, 3, 1, 3, 1),
+    ("2DXYp", 700, 3, 1, 5, 2, 5, 2),
+    ("2DYp", 1100, 4, 2, 1, 0, 7, 3),
 ]
 
 
