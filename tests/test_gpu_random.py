@@ -164,3 +164,45 @@ def test_random_weno(gpu, nx, ny, seed):
     ref = oracle.weno(phi, u, v, dx, dy, single, nthreads=8)
     got = dout.cpu().numpy()
     assert bits_equal(got, ref), mismatches(got, ref)
+
+
+@pytest.mark.parametrize("n,steps,mode", [(int(a), int(b), int(c)) for a, b, c in
+                                          np.random.default_rng(16).integers(
+                                              [8, 1, 0], [300, 5, 2], size=(8, 3))])
+def test_random_ch_sizes(gpu, n, steps, mode):
+    """Cahn-Hilliard ADI at random grid sizes (not multiples of the 64-point
+    tiles or blocks), fused and plain modes, against the oracle."""
+    c0 = synth.uniform(n * n, 7000 + n, -0.1, 0.1).reshape(n, n)
+    s = api.CHSolver(api.ch_params(n), c0)
+    s.run(steps, mode=mode)
+    c, co = s.field()
+    s.close()
+    ref = oracle.CH(n)
+    rc, rco = c0.ravel().copy(), c0.ravel().copy()
+    ref.step(rc, rco, steps=steps)
+    assert mismatches(c, rc.reshape(n, n)) == 0
+    assert mismatches(co, rco.reshape(n, n)) == 0
+
+
+@pytest.mark.parametrize("sizes,seed", [([int(x) for x in np.random.default_rng(17 + k).integers(
+    8, 400, size=int(np.random.default_rng(17 + k).integers(2, 9)))], 17 + k) for k in range(4)])
+def test_random_batch(gpu, sizes, seed):
+    """Batched grids of random uneven (non-square-multiple) sizes, fused with
+    the strip kernel and with the band walker, against the oracle."""
+    torch = gpu
+    w = synth.uniform(25, seed, -0.05, 0.05)
+    fields = [synth.field(n, n, seed * 100 + g) for g, n in enumerate(sizes)]
+    for band in (False, True):
+        ins = [torch.from_numpy(f.copy()).cuda() for f in fields]
+        outs = [torch.zeros_like(t) for t in ins]
+        b = api.Batch(outs, ins, torch.from_numpy(w).cuda())
+        b.set_tile_config(4 if band else 8, 2, band)
+        b.run(3)
+        torch.cuda.synchronize()
+        cur = outs if b.parity == 1 else ins
+        for g, f in enumerate(fields):
+            ref = f
+            for _ in range(3):
+                ref = oracle.apply(ref, w, 5, 2, 5, 2, True)
+            assert bits_equal(cur[g].cpu().numpy(), ref), (band, g, sizes[g])
+        b.close()
