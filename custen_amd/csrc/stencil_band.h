@@ -82,6 +82,31 @@ __device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, Lds<T>* l)
     __builtin_amdgcn_global_load_lds((const Glob<void>*)g, (LdsVoid*)l, 4, 0, POLICY);
 }
 
+// FP64 stencils of more than 36 points have more weights than the scalar
+// registers hold next to the walker's own state; held for the whole walk they
+// are spilled and read back lane by lane.  The unmasked (FULL) walk of those
+// stencils instead loads each weight row from the scalar cache just before
+// its FMAs (the weights are not written during a launch, so the constant
+// address space applies).  The masked walk keeps resident weights: with its
+// store masks in the loop the compiler spilled more when streaming.
+template <typename T, int H, int V, bool FULL>
+constexpr bool kStreamWeights = FULL && sizeof(T) == 8 && H * V > 36;
+
+template <int H>
+__device__ __forceinline__ void loadWeightRow(const double* weights, int j, double (&wr)[H],
+                                              double after)
+{
+    // The empty asm makes the address opaque at each use and orders it after
+    // `after` (the accumulator the row updates), so the loads stay next to
+    // the FMAs that consume them instead of being hoisted out of the walk.
+    uint64_t p = (uint64_t)weights;
+    __asm__ __volatile__("" : "+s"(p) : "v"(after));
+    typedef __attribute__((address_space(4))) const double CDouble;
+    const CDouble* q = (const CDouble*)p;
+#pragma unroll
+    for (int i = 0; i < H; i++) wr[i] = q[j * H + i];
+}
+
 // Ring geometry of a band-walker instantiation (2 columns per lane).  A row
 // body is one 16-byte load per lane for FP64 and two 4-byte loads (columns
 // lane and 64 + lane) for FP32; each halo side is one 4-byte load by the
@@ -106,7 +131,8 @@ struct BandShape {
 // columns, pair-aligned rows), and wrapped row indices need at most one
 // correction for every row the band touches.  FULL (also decided by the
 // host): every point of every walked strip is written and the region height
-// is a multiple of RPW, so each chunk stores RPW whole rows.
+// is a multiple of RPW, so each chunk stores RPW whole rows.  With
+// kStreamWeights the weights come from a.weights and `w` is not read.
 template <typename T, int H, int V, int RPW, bool FULL, Xform X = Xform::Identity>
 __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H * V],
                                          Lds<T>* __restrict__ ring, int g)
@@ -217,8 +243,7 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
             const Lds<T>* row = ring + sl * LD + (HLP - L) + 2 * lane;
             T v[H + 1];
             // Pair reads only for stencils of at most 25 points: the wider
-            // ones hold their weights in spilled scalar registers and run
-            // slower with a second read path in the loop.
+            // ones were measured slower with a second read path in the loop.
             if (H * V <= 25 && ((HLP - L) & 1) == 0 && (H + 1) % 2 == 0) {
                 // window pair-aligned: one 2-element read per pair (FP64:
                 // ds_read_b128, 16 lanes x 16 B cover all banks once)
@@ -238,10 +263,17 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
             for (int r = 0; r < RPW; r++) {
                 const int j = ir - r;
                 if (j >= 0 && j < V) {
+                    T wr[H];
+                    if constexpr (kStreamWeights<T, H, V, FULL>) {
+                        loadWeightRow<H>(a.weights, j, wr, acc[r][1]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < H; i++) wr[i] = w[j * H + i];
+                    }
 #pragma unroll
                     for (int i = 0; i < H; i++) {
-                        acc[r][0] = fmaT(w[j * H + i], v[i], acc[r][0]);
-                        acc[r][1] = fmaT(w[j * H + i], v[i + 1], acc[r][1]);
+                        acc[r][0] = fmaT(wr[i], v[i], acc[r][0]);
+                        acc[r][1] = fmaT(wr[i], v[i + 1], acc[r][1]);
                     }
                 }
             }
@@ -283,8 +315,10 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     using BS = BandShape<T, H, V, RPW>;
     __shared__ __attribute__((aligned(16))) T ring[BS::S * BS::LD];
     T w[H * V];
+    if constexpr (!kStreamWeights<T, H, V, FULL>) {
 #pragma unroll
-    for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
+        for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
+    }
     bandWalk<T, H, V, RPW, FULL, X>(a, w, (Lds<T>*)ring, xcdRemap(blockIdx.x, gridDim.x));
 }
 
