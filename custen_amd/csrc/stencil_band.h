@@ -124,8 +124,8 @@ struct BandShape {
 };
 
 // Band walk of wave g of a launch (one wave per workgroup): interior strip
-// stripTx0 + g % nStripTx over region rows [band * RPW * chunks, ...) with
-// band = g / nStripTx.  `ring` is the wave's BandShape::S x LD LDS ring and
+// stripTx0 + g % nStripTx (or, with edgeList, edgeTx[g % nStripTx]) over
+// region rows [band * RPW * chunks, ...) with band = g / nStripTx.  `ring` is the wave's BandShape::S x LD LDS ring and
 // `w` the H*V weights; X the point-wise transform of the values read.
 // Requirements (checked by the host): the strip is interior (full 128
 // columns, pair-aligned rows), and wrapped row indices need at most one
@@ -142,7 +142,7 @@ __device__ __forceinline__ void bandWalk(const StencilLaunch& a, const T (&w)[H 
     constexpr int HL = BS::HALO_LANES;
 
     const int lane = threadIdx.x;
-    const int tx = a.stripTx0 + g % a.nStripTx;
+    const int tx = a.edgeList ? a.edgeTx[g % a.nStripTx] : a.stripTx0 + g % a.nStripTx;
     const int band = g / a.nStripTx;
     const int bandRows = RPW * a.chunks;
     const int yS = band * bandRows;

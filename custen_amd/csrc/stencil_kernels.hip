@@ -13,6 +13,7 @@
 // the points whose whole window lies in the domain (see StencilLaunch).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 
 #include "kernels.h"
@@ -393,13 +394,77 @@ hipError_t runFixed(const StencilLaunch& in, hipStream_t st)
                 const bool full = lo * dev::kTW >= s.wx0 && hi * dev::kTW <= s.wx1 &&
                                   s.yGlobal0 >= s.wy0 && s.yGlobal0 + s.ny <= s.wy1 &&
                                   s.ny % RPW == 0;
-                if (full)
-                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true, X>),
-                                       dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
-                else
-                    hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false, X>),
-                                       dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
-                return hipGetLastError();
+                if (full || !dev::kStreamWeights<T, H, V, true>) {
+                    if (full)
+                        hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true, X>),
+                                           dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
+                    else
+                        hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false, X>),
+                                           dim3((unsigned)waves), dim3(dev::kLanes), 0, st, s);
+                    return hipGetLastError();
+                }
+                // Stencils whose masked walk keeps resident (spilled) weights:
+                // the unmasked walk serves the strips whose every column is
+                // written, [xa, xb), over the whole chunks of the written rows
+                // [ya, ya + yFull); the masked walk serves the ragged rows and
+                // the strips at the column window's edges.  Rows outside the
+                // row window are not written and are skipped.  A row offset
+                // needs clamped rows (wrapped rows are addressed from region
+                // row 0).
+                int xa = lo, xb = hi;
+                while (xa < xb && xa * dev::kTW < s.wx0) xa++;
+                while (xb > xa && xb * dev::kTW > s.wx1) xb--;
+                const int ya = std::max(0, s.wy0 - s.yGlobal0);
+                const int yb = std::min(s.ny, s.wy1 - s.yGlobal0);
+                int yFull = yb > ya ? (yb - ya) / RPW * RPW : 0;
+                if (s.rows.wrap && (ya != 0 || yFull != s.ny)) yFull = 0;
+                auto walk = [&](bool fullWalk, const StencilLaunch& strips, int y0,
+                                int n) -> hipError_t {
+                    if (strips.nStripTx <= 0 || n <= 0) return hipSuccess;
+                    StencilLaunch t = strips;
+                    t.ny = n;
+                    if (y0 > 0) {
+                        t.in = static_cast<const char*>(s.in) + (int64_t)y0 * s.ldIn * sizeof(T);
+                        t.out = static_cast<char*>(s.out) + (int64_t)y0 * s.ldOut * sizeof(T);
+                        t.yGlobal0 = s.yGlobal0 + y0;
+                        t.rows.rowMin = s.rows.rowMin - y0;
+                        t.rows.rowMax = s.rows.rowMax - y0;
+                    }
+                    const unsigned w =
+                        (unsigned)((int64_t)t.nStripTx * ((n + bandRows - 1) / bandRows));
+                    if (fullWalk)
+                        hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, true, X>), dim3(w),
+                                           dim3(dev::kLanes), 0, st, t);
+                    else
+                        hipLaunchKernelGGL((dev::stencilBand<T, H, V, RPW, false, X>), dim3(w),
+                                           dim3(dev::kLanes), 0, st, t);
+                    return hipGetLastError();
+                };
+                auto range = [&](int tx0, int ntx) {
+                    StencilLaunch t = s;
+                    t.edgeList = 0;
+                    t.stripTx0 = tx0;
+                    t.nStripTx = ntx;
+                    return t;
+                };
+                if (yFull == 0 || xb <= xa) return walk(false, range(lo, hi - lo), 0, s.ny);
+                hipError_t err;
+                if ((err = walk(true, range(xa, xb - xa), ya, yFull)) != hipSuccess) return err;
+                if ((err = walk(false, range(xa, xb - xa), ya + yFull, yb - ya - yFull)) !=
+                    hipSuccess)
+                    return err;
+                const int nSide = (xa - lo) + (hi - xb);
+                if (nSide <= 4) {
+                    // both edges in one launch
+                    StencilLaunch e = s;
+                    e.edgeList = 1;
+                    e.nStripTx = 0;
+                    for (int tx = lo; tx < hi; tx++)
+                        if (tx < xa || tx >= xb) e.edgeTx[e.nStripTx++] = tx;
+                    return walk(false, e, ya, yb - ya);
+                }
+                if ((err = walk(false, range(lo, xa - lo), ya, yb - ya)) != hipSuccess) return err;
+                return walk(false, range(xb, hi - xb), ya, yb - ya);
             }
         }
         hipLaunchKernelGGL((dev::stencilFixed<T, H, V, RPW, X, true, CPL>),
