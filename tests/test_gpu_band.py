@@ -97,6 +97,42 @@ def test_band_host_streaming(gpu, variant, nt, nx, ny):
     assert mismatches(hout.numpy(), ref) == 0
 
 
+@pytest.mark.parametrize("variant,H,L,V,T", [("2DXYnp", 7, 3, 7, 3), ("2DXYnp", 7, 1, 7, 5),
+                                              ("2DXYnp", 9, 4, 9, 4), ("2DXYnp", 9, 8, 9, 0)])
+@pytest.mark.parametrize("nx,ny", [(428, 101), (640, 333), (1000, 77), (256, 40)])
+@pytest.mark.parametrize("rpw,chunks", [(4, 2), (8, 1), (4, 5)])
+def test_band_window_split(gpu, variant, H, L, V, T, nx, ny, rpw, chunks):
+    """Non-periodic stencils of more than 36 points: the written window runs
+    unmasked, the edge strips (one launch, by list) and the ragged rows that
+    are left run masked, the rows outside the window are not walked.  Sizes
+    give ragged rows for every rows-per-wave (or none), one or two edge
+    strips, and a right edge inside the ragged column strip."""
+    got, ref = _run(gpu, variant, nx, ny, H, L, V, T, rpw, chunks)
+    assert mismatches(got, ref) == 0, f"{mismatches(got, ref)} points differ"
+
+
+@pytest.mark.parametrize("nt,nx,ny", [(3, 520, 150), (4, 640, 204), (5, 384, 105)])
+def test_band_window_split_host_streaming(gpu, nt, nx, ny):
+    """The window split on host-streamed tiles: each tile is a region whose
+    rows start at a global offset, so the row window starts inside the first
+    tile only and ends inside the last."""
+    torch = gpu
+    H, L, V, T = 7, 3, 7, 3
+    u = synth.field(nx, ny, 305)
+    w = synth.uniform(H * V, 306, -1.0, 1.0)
+    init = np.full_like(u, SENTINEL)
+    hin = torch.from_numpy(u).pin_memory()
+    hout = torch.from_numpy(init).pin_memory()
+    h = api.cuSten_t()
+    create_weights("2DXYnp", h, nt, nx, ny, hout, hin, w, H, L, V, T)
+    api.custenSetBandConfig(h, 4, 2)
+    api.cuStenCompute2DXYnp(h, api.HOST)
+    torch.cuda.synchronize()
+    api.cuStenDestroy2DXYnp(h)
+    ref = oracle_weights("2DXYnp", u, w, H, L, V, T, init)
+    assert mismatches(hout.numpy(), ref) == 0
+
+
 def test_band_repeated_and_swapped(gpu):
     """Several Computes with Swap in between: the ring and counters start
     clean for every wave of every launch."""
