@@ -92,6 +92,12 @@ __device__ __forceinline__ void ldsDma4(const Glob<unsigned>* g, Lds<T>* l)
 template <typename T, int H, int V, bool FULL>
 constexpr bool kStreamWeights = FULL && sizeof(T) == 8 && H * V > 36;
 
+// FP32 stencils of more than 36 points: the weights (converted from the
+// FP64 array on the vector unit) stay in vector registers, 49 or 81 of them,
+// instead of being moved to scalar registers that then spill.
+template <typename T, int H, int V>
+constexpr bool kWeightsInVgprs = sizeof(T) == 4 && H * V > 36;
+
 template <int H>
 __device__ __forceinline__ void loadWeightRow(const double* weights, int j, double (&wr)[H],
                                               double after)
@@ -318,6 +324,16 @@ __global__ __launch_bounds__(kLanes) void stencilBand(StencilLaunch a)
     if constexpr (!kStreamWeights<T, H, V, FULL>) {
 #pragma unroll
         for (int k = 0; k < H * V; k++) w[k] = static_cast<T>(a.weights[k]);
+    }
+    if constexpr (kWeightsInVgprs<T, H, V>) {
+        // an empty asm with a vector-register result: the compiler treats
+        // the value as per-lane and keeps it there
+#pragma unroll
+        for (int k = 0; k < H * V; k++) {
+            T o;
+            __asm__ __volatile__("" : "=v"(o) : "0"(w[k]));
+            w[k] = o;
+        }
     }
     bandWalk<T, H, V, RPW, FULL, X>(a, w, (Lds<T>*)ring, xcdRemap(blockIdx.x, gridDim.x));
 }
