@@ -241,6 +241,27 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
                                                    "GPts/s": round(pts / (t * 1e-3) / 1e9, 1)}
         destroy(api, kind, h)
 
+    # the largest built-in xy shapes, centred, periodic and not (synthetic
+    # weights; the autotuner picks the configuration on the first Compute)
+    from custen_amd import synth
+    for H in (7, 9):
+        wbig = torch.from_numpy(synth.uniform(H * H, 40 + H, -0.02, 0.02)).cuda()
+        for per in ("p", "np"):
+            name = "2DXYp" if per == "p" else "2DXYnp"
+            h = api.cuSten_t()
+            getattr(api, f"cuStenCreate{name}")(h, 0, 1, n, n, 32, 32, dout, din, wbig,
+                                                 H, H // 2, H // 2, H, H // 2, H // 2)
+            api.custenSetStream(h, stream.cuda_stream)
+            comp = getattr(api, f"cuStenCompute{name}")
+            fn = lambda: comp(h, api.DEVICE)  # noqa: E731
+            fn()
+            torch.cuda.synchronize()
+            t = timer.run(fn, k2, 2, prewarm_s=0.05)
+            extra[f"xy_{per}_{H}x{H}_8192"] = {"us": round(t * 1e3, 1),
+                                               "GPts/s": round(pts / (t * 1e-3) / 1e9, 1),
+                                               "GB/s": round(16.0 * pts / (t * 1e-3) / 1e9, 1)}
+            getattr(api, f"cuStenDestroy{name}")(h)
+
     # FP32 storage, error against the FP64 oracle on the first 256 rows
     din32 = din.float()
     dout32 = torch.empty_like(din32)
