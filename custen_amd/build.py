@@ -106,6 +106,16 @@ def build_oracle() -> str:
     return os.path.join(odir, "libcusten_oracle.so")
 
 
+def build_native_callers() -> str:
+    """The reference-shaped C++ callers the GPU tests run as child processes
+    (tests/native/Makefile); they link against the library built above."""
+    ndir = os.path.join(REPO, "tests", "native")
+    subprocess.run(["make", "-s", "-C", ndir, "-B", "-j2", f"HIPCC={HIPCC}", f"ARCH={ARCH}"],
+                   check=True)
+    return os.path.join(ndir, "bin")
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_oracle()
+    build_native_callers()

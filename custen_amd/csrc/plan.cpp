@@ -137,23 +137,27 @@ double* offsetPtr(void* base, int64_t elems, Dtype d)
 }
 
 // Halo pointer tables of the y / xy variants, from a given input base.
-void setBoundaries(cuSten_t* h, const Plan* p, void* in)
+// Two layouts, as the reference sets them:
+//   wrapped  tile 0's top halo starts at row ny-T and the last tile's bottom
+//            halo at row 0; tile t's at t*nyTile-T and (t+1)*nyTile.  Create
+//            and Swap of every periodic variant, Create of 2DXYnp(Fun), and
+//            Swap of every non-periodic variant
+//            (custenCreateDestroy2DXYp.cu:194-228, 2DXYnp.cu:196-228,
+//            2DYnp.cu:248-279, 2DXYnp.cu:265-290).
+//   clamped  tile 0's top halo at row 0 and the last tile's bottom halo at
+//            row ny-B: Create of 2DYnp(Fun) (custenCreateDestroy2DYnp.cu:183-214,
+//            2DYnpFun.cu:186-216).
+// The kernels never read these tables (halos are resolved from the whole
+// grid), so the two layouts only differ in these observable fields.
+void setBoundaries(cuSten_t* h, const Plan* p, void* in, bool clamped)
 {
     if (h->boundaryTop == nullptr || h->boundaryBottom == nullptr) return;
     const int64_t nx = h->nx, ny = h->ny, nyT = h->nyTile;
     const int nt = h->numTiles;
     for (int t = 0; t < nt; t++) {
-        int64_t top, bot;
-        if (p->periodic) {
-            // custenCreateDestroy2DXYp.cu:194-228 (and 2DYp)
-            top = ((int64_t)t * nyT - p->T) % ny;
-            if (top < 0) top += ny;
-            bot = ((int64_t)(t + 1) * nyT) % ny;
-        } else {
-            // custenCreateDestroy2DYnp.cu:183-214
-            top = (t == 0) ? 0 : (int64_t)t * nyT - p->T;
-            bot = (t == nt - 1) ? ny - (p->V - 1 - p->T) : (int64_t)(t + 1) * nyT;
-        }
+        int64_t top = (t == 0) ? (clamped ? 0 : ny - p->T) : (int64_t)t * nyT - p->T;
+        if (top < 0) top = ((top % ny) + ny) % ny;  // halo taller than a tile
+        int64_t bot = (t == nt - 1) ? (clamped ? ny - (p->V - 1 - p->T) : 0) : (int64_t)(t + 1) * nyT;
         h->boundaryTop[t] = offsetPtr(in, top * nx, p->dtype);
         h->boundaryBottom[t] = offsetPtr(in, bot * nx, p->dtype);
     }
@@ -376,7 +380,7 @@ void createCommon(cuSten_t* h, const CreateArgs& a)
     if (halos) {
         h->numBoundaryTop = a.T * a.nx;
         h->numBoundaryBottom = a.B * a.nx;
-        setBoundaries(h, p, a.in);
+        setBoundaries(h, p, a.in, !a.periodic && a.dir == Dir::Y);
     }
 }
 
@@ -837,7 +841,7 @@ void swapCommon(cuSten_t* h, void* dataInput, const char* fn)
         setError(CUSTEN_ERROR_INVALID_VALUE,
                  std::string(fn) + ": dataInput is not the swapped input base; halo pointers "
                                    "follow the argument, tiles follow the swap");
-    setBoundaries(h, p, dataInput);
+    setBoundaries(h, p, dataInput, false);
 }
 
 void destroyCommon(cuSten_t* h, const char* fn)
@@ -1279,18 +1283,24 @@ void custenClearError(void)
 const char* custenVersion(void) { return "custen-amd 0.1.0 (gfx950)"; }
 size_t custenHandleSize(void) { return sizeof(cuSten_t); }
 
-// util.h:43 / error.cu:41-50: print and exit on a pending error
-void checkError(const char* msg)
+// util.h:43 / error.cu:41-50: on a pending HIP error, print
+// "\nError while '<action>': <error string>\nprogram terminated ...\n\n" to
+// stdout and exit(EXIT_FAILURE).  An error this library recorded (a refused
+// Create, a failed Compute) is reported and ends the process the same way;
+// the reference had no such errors because it validated nothing.
+void checkError(const char* action)
 {
+    const char* what = action ? action : "";
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-        std::printf("%s \n", msg ? msg : "");
-        std::printf("Error: %s\n", hipGetErrorString(e));
+        std::printf("\nError while '%s': %s\nprogram terminated ...\n\n", what,
+                    hipGetErrorString(e));
+        std::fflush(stdout);
         std::exit(EXIT_FAILURE);
     }
     if (g_err != CUSTEN_SUCCESS) {
-        std::printf("%s \n", msg ? msg : "");
-        std::printf("Error: %s\n", g_msg.c_str());
+        std::printf("\nError while '%s': %s\nprogram terminated ...\n\n", what, g_msg.c_str());
+        std::fflush(stdout);
         std::exit(EXIT_FAILURE);
     }
 }

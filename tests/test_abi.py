@@ -141,3 +141,37 @@ def test_ch_restore_rejects_bad_files(tmp_path):
         api.CHSolver.restore(str(short))
     with pytest.raises(api.CustenError, match="cannot open"):
         api.CHSolver.restore(str(tmp_path / "missing.ckpt"))
+
+
+def test_check_error_prints_and_exits(tmp_path):
+    """checkError after a refused Create prints error.cu:46's text,
+    "\\nError while '<action>': <error>\\nprogram terminated ...\\n\\n", and
+    exits with EXIT_FAILURE (error.cu:41-50).  Run in a child process."""
+    import re
+    import subprocess
+    import sys
+
+    script = tmp_path / "child.py"
+    script.write_text(
+        "import ctypes, sys\n"
+        f"sys.path.insert(0, {api._HERE!r} + '/..')\n"
+        "from custen_amd import api\n"
+        "L = api.lib()\n"
+        "h = api.cuSten_t()\n"
+        "L.cuStenCreate2DXYp(ctypes.byref(h), 0, 1, 0, 64, 8, 8, 1, 2, 3, 5, 2, 2, 5, 2, 2)\n"
+        "sys.stderr.write(L.custenGetLastErrorString().decode() + '\\n')\n"
+        "sys.stderr.flush()\n"
+        "L.checkError(b'creating the handle')\n"
+        "sys.exit(0)\n")
+    res = subprocess.run([sys.executable, str(script)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 1, (res.returncode, res.stdout, res.stderr)
+    m = re.fullmatch(r"\nError while 'creating the handle': (.+)\nprogram terminated \.\.\.\n\n",
+                     res.stdout)
+    assert m, repr(res.stdout)
+    # either the library's own record of the refused Create or, on a host
+    # without a GPU, the HIP runtime's pending error
+    recorded = res.stderr.strip().splitlines()[-1]
+    assert "nx and ny must be >= 1" in recorded
+    assert m.group(1) == recorded or "device" in m.group(1).lower()
+

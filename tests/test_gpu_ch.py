@@ -33,8 +33,15 @@ def test_solver_matches_oracle(gpu, n, steps, mode):
     assert mismatches(co, rco.reshape(n, n)) == 0
 
 
-def test_mass_conservation_200_steps(gpu):
-    n = 256
+@pytest.mark.parametrize("n,steps", [(1024, 3), (2048, 2)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_solver_matches_oracle_config_sizes(gpu, n, steps, mode):
+    """The two largest grid sizes of the served workload, both step modes."""
+    test_solver_matches_oracle(gpu, n, steps, mode)
+
+
+@pytest.mark.parametrize("n", [256, 2048])
+def test_mass_conservation_200_steps(gpu, n):
     c0 = ic(n, 1)
     s = api.CHSolver(api.ch_params(n), c0)
     s.run(200)
@@ -56,6 +63,31 @@ def test_server_alone_vs_mix(gpu):
         assert srv.query(i)[0] == 2
     srv.close()
     for r, (n, k, c) in enumerate(zip(sizes, steps, ics)):
+        alone = api.CHServer(budget_bytes=api.ch_footprint(n), max_slots=1)
+        rid = alone.submit(api.ch_params(n), k, c)
+        alone.run_until_idle()
+        assert bits_equal(alone.result(rid), mix[r])
+        alone.close()
+        ref = oracle.CH(n)
+        rc, rco = c.ravel().copy(), c.ravel().copy()
+        ref.step(rc, rco, steps=k)
+        assert mismatches(mix[r], rc.reshape(n, n)) == 0
+
+
+def test_server_alone_vs_mix_config_sizes(gpu):
+    """A mix that includes the served workload's 1024 and 2048 grids: each
+    result is the same alone and in the mix, and equals the oracle."""
+    sizes = [2048, 64, 1024, 256, 16, 512]
+    steps = [2, 5, 3, 4, 6, 2]
+    ics = [ic(n, 40 + r) for r, n in enumerate(sizes)]
+    budget = sum(api.ch_footprint(n) for n in sizes)
+    srv = api.CHServer(budget_bytes=budget, max_slots=4)
+    ids = [srv.submit(api.ch_params(n), k, c) for n, k, c in zip(sizes, steps, ics)]
+    srv.run_until_idle()
+    mix = [srv.result(i) for i in ids]
+    srv.close()
+    for r, (n, k, c) in enumerate(zip(sizes, steps, ics)):
+        print(f"alone vs mix: n={n} steps={k}", flush=True)
         alone = api.CHServer(budget_bytes=api.ch_footprint(n), max_slots=1)
         rid = alone.submit(api.ch_params(n), k, c)
         alone.run_until_idle()

@@ -237,3 +237,24 @@ def test_f32_wide_shapes(gpu, H, L, V, T):
     ref = oracle.apply(u, w.astype(np.float32), H, L, V, T, True)
     got = dout.cpu().numpy()
     assert bits_equal(got, ref), mismatches(got, ref)
+
+
+@pytest.mark.parametrize("nt", [16])
+def test_host_streaming_config3_width(gpu, nt):
+    """Config 3's row width (65536 columns) and tile count (numTiles = 16),
+    xy periodic 9-point, host-pinned input and output streamed through HBM:
+    bit for bit the oracle.  512 rows keep the oracle under a second."""
+    torch = gpu
+    nx, ny = 65536, 512
+    u = synth.field(nx, ny, 3)
+    w = synth.uniform(9, 31, -1.0, 1.0)
+    hin = torch.from_numpy(u).pin_memory()
+    hout = torch.zeros(ny, nx, dtype=torch.float64).pin_memory()
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYp(h, 0, nt, nx, ny, 32, 32, hout, hin, w, 3, 1, 1, 3, 1, 1)
+    assert h.nyTile == ny // nt
+    api.cuStenCompute2DXYp(h, api.HOST)
+    torch.cuda.synchronize()
+    api.cuStenDestroy2DXYp(h)
+    ref = oracle.apply(u, w, 3, 1, 3, 1, True, nthreads=8)
+    assert mismatches(hout.numpy(), ref) == 0
