@@ -188,6 +188,8 @@ _SIGS.update({
     "custenCHServerSubmitResume": (c_int, [c_void_p, ctypes.POINTER(custenCHParams), c_int, _P,
                                            _P]),
     "custenCHServerDestroy": (None, [c_void_p]),
+    "custenCHAdvanceNoAlloc": (c_int, [c_int, ctypes.POINTER(custenCHParams), _P, _P, c_int,
+                                       c_void_p]),
 })
 
 _lib = None
@@ -493,6 +495,18 @@ class CHSolver:
             self.close()
         except Exception:
             pass
+
+
+def ch_advance_no_alloc(params: custenCHParams, c, c_old, steps: int, stream: int = 0,
+                        device: int = 0) -> None:
+    """custenCHAdvanceNoAlloc: advance the simulation held in the device
+    arrays c (current) and c_old (previous) by `steps` fused steps without any
+    device allocation (n <= 1024)."""
+    rc = lib().custenCHAdvanceNoAlloc(device, ctypes.byref(params), _addr(c), _addr(c_old), steps,
+                                      stream)
+    if rc != 0:
+        check("custenCHAdvanceNoAlloc")
+        raise CustenError(rc, "custenCHAdvanceNoAlloc failed")
 
 
 class CHServer:

@@ -27,6 +27,7 @@
 #include "../../include/custen_ch.h"
 #include "ch_kernels.h"
 #include "ch_plain.h"
+#include "ch_setup.h"
 #include "plan_internal.h"
 
 #pragma STDC FP_CONTRACT OFF
@@ -46,48 +47,8 @@ struct CHCoeffs {
     std::vector<double> fac;  // ds, dl, d, du, dw, inv1, inv2
 };
 
-void pentFactor(double* ds, double* dl, double* d, double* du, double* dw, int m)
-{
-    // serialCahnADI.c:91-155 / cuPentBatch.cu:35-113
-    int r = 0;
-    du[r] = du[r] / d[r];
-    dw[r] = dw[r] / d[r];
-    int rp = r;
-    r += 1;
-    d[r] = d[r] - dl[r] * du[rp];
-    du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
-    dw[r] = dw[r] / d[r];
-    for (int i = 2; i < m - 2; i++) {
-        const int rpp = r - 1;
-        rp = r;
-        r += 1;
-        dl[r] = dl[r] - ds[r] * du[rpp];
-        d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
-        dw[r] = dw[r] / d[r];
-        du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
-    }
-    for (int last = 0; last < 2; last++) {
-        const int rpp = r - 1;
-        rp = r;
-        r += 1;
-        dl[r] = dl[r] - ds[r] * du[rpp];
-        d[r] = d[r] - ds[r] * dw[rpp] - dl[r] * du[rp];
-        if (last == 0) du[r] = (du[r] - dl[r] * dw[rp]) / d[r];
-    }
-}
-
-void pentSolve(const double* ds, const double* dl, const double* d, const double* du,
-               const double* dw, double* b, int m)
-{
-    // serialCahnADI.c:161-241 / cuPentBatch.cu:119-198
-    b[0] = b[0] / d[0];
-    b[1] = (b[1] - dl[1] * b[0]) / d[1];
-    for (int i = 2; i < m; i++) b[i] = (b[i] - ds[i] * b[i - 2] - dl[i] * b[i - 1]) / d[i];
-    b[m - 2] = b[m - 2] - du[m - 2] * b[m - 1];
-    for (int i = m - 3; i >= 0; i--) b[i] = b[i] - du[i] * b[i + 1] - dw[i] * b[i + 2];
-}
-
-bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
+// Validation and the constants of the scheme (everything but the factors).
+bool chConstants(const custenCHParams& p, CHCoeffs& co, std::string& why)
 {
     if (p.n < 8 || p.n > kCHMaxN) {
         why = "n must be in [8, " + std::to_string(kCHMaxN) + "]";
@@ -116,41 +77,14 @@ bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
     static const double nl[9] = {0, 1, 0, 1, -4, 1, 0, 1, 0};
     for (int k = 0; k < 25; k++) co.wlin[k] = lin[k] * s;
     for (int k = 0; k < 9; k++) co.wnl[k] = nl[k] * co.snl;
-    co.fac.assign((size_t)7 * m, 0.0);
-    double* ds = co.fac.data();
-    double* dl = ds + m;
-    double* dd = dl + m;
-    double* du = dd + m;
-    double* dw = du + m;
-    double* inv1 = dw + m;
-    double* inv2 = inv1 + m;
-    for (int i = 0; i < m; i++) {
-        if (i > 1) ds[i] = co.a;
-        if (i > 0) dl[i] = co.b;
-        dd[i] = co.c;
-        if (i < m - 1) du[i] = co.d;
-        if (i < m - 2) dw[i] = co.e;
-    }
-    pentFactor(ds, dl, dd, du, dw, m);
-    inv1[0] = co.a;
-    inv1[m - 2] = co.e;
-    inv1[m - 1] = co.d;
-    inv2[0] = co.b;
-    inv2[1] = co.a;
-    inv2[m - 1] = co.e;
-    pentSolve(ds, dl, dd, du, dw, inv1, m);
-    pentSolve(ds, dl, dd, du, dw, inv2, m);
-    const double a = co.a, b = co.b, c = co.c, d = co.d, e = co.e;
-    const double z11 = e * inv1[0] + a * inv1[m - 2] + b * inv1[m - 1];
-    const double z12 = e * inv2[0] + a * inv2[m - 2] + b * inv2[m - 1];
-    const double z21 = d * inv1[0] + e * inv1[1] + a * inv1[m - 1];
-    const double z22 = d * inv2[0] + e * inv2[1] + a * inv2[m - 1];
-    const double y11 = c - z11, y12 = d - z12, y21 = b - z21, y22 = c - z22;
-    const double det = 1.0 / (y11 * y22 - y21 * y12);
-    co.omega[0] = y22 * det;
-    co.omega[1] = -y12 * det;
-    co.omega[2] = -y21 * det;
-    co.omega[3] = y11 * det;
+    return true;
+}
+
+bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
+{
+    if (!chConstants(p, co, why)) return false;
+    co.fac.assign((size_t)7 * co.m, 0.0);
+    chFactor(co.fac.data(), co.m, co.a, co.b, co.c, co.d, co.e, co.omega);
     return true;
 }
 
@@ -384,6 +318,23 @@ private:
 };
 
 }  // namespace
+
+bool chDescribe(const custenCHParams& p, CHDesc& d, double& cDiag, std::string& why)
+{
+    CHCoeffs co;
+    if (!chConstants(p, co, why)) return false;
+    std::memset(&d, 0, sizeof(d));
+    d.a = co.a;
+    d.b = co.b;
+    d.d = co.d;
+    d.e = co.e;
+    for (int k = 0; k < 25; k++) d.wlin[k] = co.wlin[k];
+    for (int k = 0; k < 9; k++) d.wnl[k] = co.wnl[k];
+    d.n = co.n;
+    cDiag = co.c;
+    return true;
+}
+
 }  // namespace custen
 
 using namespace custen;

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <string>
+
+#include "../../include/custen_ch.h"
 
 #if defined(__HIPCC__)
 #define CUSTEN_HD __host__ __device__
@@ -59,5 +62,10 @@ hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, int
 hipError_t chLaunchColSolve(const CHDesc* descs, int count, int totalColBlocks, int which,
                             hipStream_t st);
 hipError_t chLaunchFinal(const CHDesc* descs, int count, int totalTiles, int step, hipStream_t st);
+
+// Validates `p` and fills the constants of a descriptor: a, b, d, e, the two
+// weight stencils and n (pointers, omega and the work bases are left zero);
+// cDiag is the main diagonal 1 + 6 sigma.  False with `why` when invalid.
+bool chDescribe(const custenCHParams& p, CHDesc& d, double& cDiag, std::string& why);
 
 }  // namespace custen

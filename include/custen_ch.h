@@ -81,6 +81,24 @@ int custenCHServerQuery(custenCHServer* s, int id, int* state, double* latencySu
                         double* latencyAdmit);
 /* Copy a finished request's final field to host memory (n*n doubles). */
 int custenCHServerResult(custenCHServer* s, int id, double* c);
+/* Allocation-free advance.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Advances the simulation held in c (current field) and cOld
+ * (previous field), both n*n doubles in device memory of deviceNum, by
+ * `steps` fused steps on `stream`, with the same results as custenCHRun mode 0
+ * (cOld = c at the start of a simulation, as in the reference).  On return
+ * (asynchronously, in stream order) c holds the new current field and cOld the
+ * previous one.  The work buffers (two n*n fields, the factored diagonals and
+ * the descriptor, 16.1 MiB) are a static device array of the library, one per
+ * GPU, held until the process ends; n is limited to [8, 1024] by its size and
+ * larger n is refused with CUSTEN_ERROR_INVALID_VALUE.  The call allocates
+ * nothing and copies nothing from the host.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race and must be serialised by the
+ * caller.  Returns CUSTEN_SUCCESS or an error code (also recorded for
+ * custenGetLastError). */
+int custenCHAdvanceNoAlloc(int deviceNum, const custenCHParams* params, double* c, double* cOld,
+                           int steps, hipStream_t stream);
+
 /* Bytes a request of grid size n keeps resident. */
 size_t custenCHFootprint(int n);
 void custenCHServerDestroy(custenCHServer* s);
