@@ -15,8 +15,9 @@ grids against the per-grid loop, batched pentadiagonal factor and solve, and
 the served Cahn-Hilliard stream against the plain per-request loop.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-  python bench.py --workload stream    # 65536 x 32768 host-resident tiled streaming
-  python bench.py --workload weno|pent|batch|ch   # one secondary benchmark alone
+  python bench.py --workload stream    # 65536 x 32768 host-resident tiled streaming alone
+                                       # (also part of the default run's "extra")
+  python bench.py --workload weno|pent|batch|ch|x1024   # one secondary benchmark alone
 """
 from __future__ import annotations
 
@@ -200,16 +201,47 @@ def bench_stencil_8192(args, torch, api, extra):
     din.copy_(torch.from_numpy(host))  # the variants below start from the original field
     log(f"xy_p_bih 8192^2: {ms * 1e3:.1f} us/apply")
     # torch's device-to-device copy of the same 512 MiB, for comparison (also
-    # the calibration dispatch for the rocprof traffic counters); it is not the
-    # copy ceiling: a grid-stride copy with non-temporal loads and stores runs
-    # at about 6.3 TB/s on this part
+    # the calibration dispatch for the rocprof traffic counters)
     tc = timer.run(lambda: dout.copy_(din), max(5, args.steps // 4), 2, prewarm_s=0.05)
     extra["torch_copy_GBps"] = round(2 * 8.0 * pts / (tc * 1e-3) / 1e9, 1)
+    extra["copy_ceiling"] = copy_ceiling(args, torch, api, timer, din, dout, ms)
     if not args.no_extra:
         bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref)
     del din, dout
     torch.cuda.empty_cache()
     return ms
+
+
+def copy_ceiling(args, torch, api, timer, din, dout, headline_ms):
+    """The same time-stepping loop (apply, then swap) with a 1 x 1 stencil of
+    weight 1.0: a copy of the 8192^2 field through the library's own kernels
+    (the autotuner picks the configuration, in practice the band walker), so
+    it moves the headline's 16 B/point with none of its arithmetic.  The
+    headline's share of this rate is reported beside it."""
+    n = din.shape[0]
+    pts = n * n
+    stream = torch.cuda.current_stream()
+    w1 = torch.ones(1, dtype=torch.float64, device="cuda")
+    h = api.cuSten_t()
+    api.cuStenCreate2DXYp(h, 0, 1, n, n, 32, 32, dout, din, w1, 1, 0, 0, 1, 0, 0)
+    api.custenSetStream(h, stream.cuda_stream)
+    api.cuStenCompute2DXYp(h, api.DEVICE)  # autotunes
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(dout, din))
+    step = timestep_fn(api, "xy_p_bih", h, din, dout)
+    t = timer.run(step, args.steps, args.warmup, prewarm_s=0.05)
+    rpw, chunks, _ = api.custenGetTileConfig(h)
+    band = api.custenGetBandMode(h)
+    api.cuStenDestroy2DXYp(h)
+    rate = 16.0 * pts / (t * 1e-3) / 1e9
+    head = 16.0 * pts / (headline_ms * 1e-3) / 1e9
+    log(f"copy ceiling (1x1 apply): {t * 1e3:.1f} us, {rate:.0f} GB/s")
+    return {"us": round(t * 1e3, 1), "GBps": round(rate, 1),
+            "frac_of_hbm_peak": round(rate / HBM_PEAK_GBS, 4),
+            "headline_frac_of_copy_ceiling": round(head / rate, 4),
+            "kernel": "band walker" if band else "strip", "rows_per_wave": rpw,
+            "chunks": chunks, "output_equals_input": exact,
+            "what": "xy-periodic 1x1 stencil, weight 1.0, 8192^2 FP64, apply + swap loop"}
 
 
 def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
@@ -439,12 +471,19 @@ def bench_x_1024(args, torch, api, extra):
     tg = Timer(torch, gs).run(lambda: graph.replay(), 10, 2) / 100
     res["graph_us"] = round(tg * 1e3, 2)
     res["graph_effective_GB/s"] = round(16.0 * n * n / (tg * 1e-3) / 1e9, 1)
-    # the graph's output equals one plain apply (same input every time)
-    ref = dout.clone()
+    # the graph's output equals one plain apply (same input every time): the
+    # eager result goes to its own buffer, the graph's output buffer is
+    # filled with NaN before one more replay, so an empty or wrong graph fails
     api.custenSetStream(h, stream.cuda_stream)
     api.cuStenCompute2DXp(h, api.DEVICE)
     torch.cuda.synchronize()
-    res["graph_matches_eager"] = bool(torch.equal(ref, dout))
+    eager = dout.clone()
+    dout.fill_(float("nan"))
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    res["graph_matches_eager"] = bool(torch.equal(eager, dout))
+    res["graph_applies_per_replay"] = 100
     del graph
     api.cuStenDestroy2DXp(h)
     extra["x_p_5pt_1024"] = res
@@ -562,10 +601,14 @@ def bench_ch(args, torch, api, extra):
             mass = max(mass, abs(res.mean() - ics[r].mean()))
         srv.close()
         lat = np.array(lat)
+        # a Poisson stream: the completion rate is bounded by the offered
+        # rate (the makespan spans the arrivals), so it measures the arrival
+        # stream, not the server's capacity (served_saturated below)
         out[f"served_load{load}"] = {
             "offered_rate_per_s": round(load * plain_rate, 2),
-            "solves_per_s": round(count / makespan, 2),
-            "speedup_vs_plain": round((count / makespan) / plain_rate, 2),
+            "arrival_limited_solves_per_s": round(count / makespan, 2),
+            "last_arrival_s": round(arrivals[-1][0], 4),
+            "makespan_s": round(makespan, 4),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "mismatching_points_vs_plain": mism,
@@ -580,8 +623,10 @@ def bench_ch(args, torch, api, extra):
     srv.run_until_idle()
     t_sat = time.perf_counter() - t0
     srv.close()
+    # capacity: every request submitted at t = 0
     out["served_saturated"] = {"solves_per_s": round(count / t_sat, 2),
-                               "speedup_vs_plain": round((count / t_sat) / plain_rate, 2)}
+                               "speedup_vs_plain": round((count / t_sat) / plain_rate, 2),
+                               "what": "capacity: all requests submitted at once"}
     out["cpu_oracle"] = ch_cpu_baseline(reqs, ics, steps)
     extra["served_cahn_hilliard"] = out
 
@@ -630,9 +675,10 @@ def host_cpu() -> dict:
     return {"model": model, "logical_cpus": os.cpu_count()}
 
 
-def bench_stream(args):
+def bench_stream(args, extra=None):
     """65536 x 32768 FP64 xy-periodic 9-point stencil with host-resident pinned
-    in/out, streamed through HBM in numTiles strips."""
+    in/out, streamed through HBM in numTiles strips.  With `extra`, the figures
+    go there; otherwise they are printed as one JSON line."""
     import torch
 
     import oracle
@@ -711,7 +757,12 @@ def bench_stream(args):
         r = res[f"numTiles{nt}"]
         # share of the host-link bound (all bytes at the measured aggregate rate)
         r["host_link_efficiency"] = round((16.0 * pts / 1e9 / agg) / r["s"], 3)
-    print(json.dumps({"workload": "stream", **res}), flush=True)
+    del dev_a, dev_b, src, dst, hin, hout, hin_np, out_np
+    torch.cuda.empty_cache()
+    if extra is not None:
+        extra["stream_65536x32768"] = res
+    else:
+        print(json.dumps({"workload": "stream", **res}), flush=True)
 
 
 def main():
@@ -720,11 +771,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="stencil",
-                    choices=["stencil", "stream", "weno", "pent", "batch", "ch"],
+                    choices=["stencil", "stream", "weno", "pent", "batch", "ch", "x1024"],
                     help="stencil: the headline line (default); the others run one "
                          "secondary benchmark alone and print its figures")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the 65536 x 32768 host-streamed configuration (34 GB of "
+                         "pinned host memory) in the default run")
     ap.add_argument("--tile", default="",
                     help="rows_per_wave,chunks,columns_per_lane of the strip kernel, or "
                          "band:rows_per_wave,chunks_per_band for the band walker, for the "
@@ -741,7 +795,7 @@ def main():
     if args.workload == "stream":
         bench_stream(args)
         return
-    if args.workload in ("weno", "pent", "batch", "ch"):
+    if args.workload in ("weno", "pent", "batch", "ch", "x1024"):
         import torch
 
         from custen_amd import api
@@ -760,6 +814,8 @@ def main():
             bench_pent(args, torch, api, extra)
         elif args.workload == "ch":
             bench_ch(args, torch, api, extra)
+        elif args.workload == "x1024":
+            bench_x_1024(args, torch, api, extra)
         else:
             bench_batch(args, torch, api, extra)
         print(json.dumps(extra), flush=True)
@@ -777,6 +833,8 @@ def main():
         bench_batch(args, torch, api, extra)
         bench_pent(args, torch, api, extra)
         bench_ch(args, torch, api, extra)
+        if not args.no_stream:
+            bench_stream(args, extra)
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         try:

@@ -492,9 +492,11 @@ hipError_t runGeneric(const StencilLaunch& s, hipStream_t st)
 }
 
 // Shapes with a compiled fixed kernel: x: H in {3,5,7,9}; y: V in {3,5,7,9};
-// xy: 3x3, 5x5.  RPW in {4, 8}.
+// xy: 3x3, 5x5; and 1x1 (a scaled copy: the walkers' own memory ceiling).
+// RPW in {4, 8}.
 #define CUSTEN_FIXED_SHAPES(X_) \
-    X_(3, 1) X_(5, 1) X_(7, 1) X_(9, 1) X_(1, 3) X_(1, 5) X_(1, 7) X_(1, 9) X_(3, 3) X_(5, 5)
+    X_(3, 1) X_(5, 1) X_(7, 1) X_(9, 1) X_(1, 3) X_(1, 5) X_(1, 7) X_(1, 9) X_(3, 3) X_(5, 5) \
+    X_(1, 1)
 #define CUSTEN_WIDE_SHAPES(X_) \
     X_(11, 1) X_(13, 1) X_(1, 11) X_(1, 13) X_(7, 7) X_(9, 9) X_(3, 5) X_(5, 3)
 

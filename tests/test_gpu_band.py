@@ -29,6 +29,8 @@ SHAPES = [
     # one-sided windows: all of the halo on one side of the strip or band
     ("2DXYp", 5, 4, 5, 0), ("2DXYnp", 3, 2, 5, 0), ("2DXp", 7, 6, 1, 0), ("2DXnp", 9, 0, 1, 0),
     ("2DYp", 1, 0, 7, 0), ("2DYnp", 1, 0, 9, 8),
+    # 1 x 1: a scaled copy through the walker
+    ("2DXYp", 1, 0, 1, 0), ("2DXYnp", 1, 0, 1, 0),
 ]
 
 

@@ -86,6 +86,10 @@ CASES = [
     ("2DXYnp", 777, 50, 1, 3, 1, 3, 1),
     ("2DXYp", 700, 3, 1, 5, 2, 5, 2),
     ("2DYp", 1100, 4, 2, 1, 0, 7, 3),
+    # 1 x 1 (a scaled copy)
+    ("2DXYp", 300, 100, 1, 1, 0, 1, 0),
+    ("2DXYnp", 256, 64, 2, 1, 0, 1, 0),
+    ("2DXp", 1024, 64, 1, 1, 0, 1, 0),
 ]
 
 

@@ -165,6 +165,14 @@ def test_fun_weighted_sums_equal_weight_stencils():
     a = oracle.apply_fun(oracle.FUN_Y_WSUM, 1, u, c9, 1, 0, 9, 4, True)
     b = oracle.apply(u, c9, 1, 0, 9, 4, True)
     assert bits_equal(a, b)
+    # x weighted sum over numCoe points starting L left of the centre (the
+    # native caller's devArg1X function), centred and one-sided
+    c5 = synth.uniform(5, seed=34, a=-1, b=1)
+    for L in (2, 1, 4):
+        for periodic in (True, False):
+            a = oracle.apply_fun(oracle.FUN_X_WSUM, 0, u, c5, 5, L, 1, 0, periodic)
+            b = oracle.apply(u, c5, 5, L, 1, 0, periodic)
+            assert bits_equal(a, b), (L, periodic)
 
 
 # ---------------------------------------------------------------------------
