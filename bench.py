@@ -17,7 +17,7 @@ the served Cahn-Hilliard stream against the plain per-request loop.
   python bench.py [--gpus 1] [--steps K] [--warmup W]
   python bench.py --workload stream    # 65536 x 32768 host-resident tiled streaming alone
                                        # (also part of the default run's "extra")
-  python bench.py --workload weno|pent|batch|ch|x1024   # one secondary benchmark alone
+  python bench.py --workload weno|pent|batch|ch|x1024|fun   # one secondary benchmark alone
 """
 from __future__ import annotations
 
@@ -315,6 +315,62 @@ def bench_stencil_variants(args, torch, api, extra, din, dout, wb, ref):
 
     bench_weno_8192(torch, api, stream, timer, din, dout, k2, extra)
 
+
+
+def bench_fun_8192(args, torch, api, extra):
+    """The point-function ("Fun") variants on the 8192^2 field, xy periodic,
+    3 x 3: the built-ins run inlined on the stencil kernels; through their
+    function pointer (forced) they take the call path a caller's own function
+    takes; and the native caller in tests/native (its own __device__ functions,
+    linked -fgpu-rdc against libcusten.a) times its functions the same way."""
+    import subprocess
+
+    from custen_amd import workloads
+
+    n = 8192
+    pts = n * n
+    stream = torch.cuda.current_stream()
+    timer = Timer(torch, stream)
+    host = np.empty((n, n))
+    api.custenFillUniform(host, pts, 2, -1.0, 1.0)
+    din = torch.from_numpy(host).cuda()
+    dout = torch.empty_like(din)
+    wl = torch.from_numpy(workloads.laplacian9_weights(n)).cuda()
+    res = {}
+    k = max(5, args.steps // 4)
+    for name in ("xy_cube_minus", "xy_weighted_sum"):
+        f = api.custenGetBuiltinFunction(name, 0)
+        for indirect in (False, True):
+            h = make_handle(api, stream, "xy_p_fun", n, dout, din, wl, func=f)
+            api.custenSetForceIndirect(h, indirect)
+            fn = compute_fn(api, "xy_p_fun", h)
+            fn()
+            torch.cuda.synchronize()
+            t = timer.run(fn, k, 2, prewarm_s=0.05)
+            destroy(api, "xy_p_fun", h)
+            res[f"{name}_{'pointer_call' if indirect else 'inlined'}"] = {
+                "us": round(t * 1e3, 1), "GPts/s": round(pts / (t * 1e-3) / 1e9, 1)}
+    del din, dout
+    torch.cuda.empty_cache()
+    exe = os.path.join(ROOT, "tests", "native", "bin", "fun_caller")
+    if os.path.exists(exe):
+        try:
+            out = subprocess.run([exe, "time", str(n), "20"], capture_output=True, text=True,
+                                 timeout=300)
+            res["native_caller_own_functions"] = json.loads(out.stdout.strip().splitlines()[-1])
+        except (subprocess.SubprocessError, ValueError, IndexError) as e:
+            res["native_caller_own_functions"] = {"error": str(e)}
+    exe = os.path.join(ROOT, "tests", "native", "bin", "point_caller")
+    if os.path.exists(exe):
+        try:
+            out = subprocess.run([exe, "time", str(n), "20"], capture_output=True, text=True,
+                                 timeout=300)
+            res["native_caller_functions_compiled_in"] = json.loads(
+                out.stdout.strip().splitlines()[-1])
+        except (subprocess.SubprocessError, ValueError, IndexError) as e:
+            res["native_caller_functions_compiled_in"] = {"error": str(e)}
+    extra["xy_p_fun_8192_paths"] = res
+    log(f"fun 8192^2: {res}")
 
 
 def bench_weno_8192(torch, api, stream, timer, din, dout, steps, extra):
@@ -771,7 +827,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="stencil",
-                    choices=["stencil", "stream", "weno", "pent", "batch", "ch", "x1024"],
+                    choices=["stencil", "stream", "weno", "pent", "batch", "ch", "x1024", "fun"],
                     help="stencil: the headline line (default); the others run one "
                          "secondary benchmark alone and print its figures")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -795,7 +851,7 @@ def main():
     if args.workload == "stream":
         bench_stream(args)
         return
-    if args.workload in ("weno", "pent", "batch", "ch", "x1024"):
+    if args.workload in ("weno", "pent", "batch", "ch", "x1024", "fun"):
         import torch
 
         from custen_amd import api
@@ -816,6 +872,8 @@ def main():
             bench_ch(args, torch, api, extra)
         elif args.workload == "x1024":
             bench_x_1024(args, torch, api, extra)
+        elif args.workload == "fun":
+            bench_fun_8192(args, torch, api, extra)
         else:
             bench_batch(args, torch, api, extra)
         print(json.dumps(extra), flush=True)

@@ -489,6 +489,33 @@ hipError_t launchRegion(const cuSten_t* h, const Plan* p, const Region& r, const
         }
         return launchStencil(s, st);
     }
+    if (p->pointLauncher) {
+        // the caller's kernel with its point function inlined (custen_point.h)
+        custenPointArgs pa{};
+        pa.in = static_cast<const double*>(r.in);
+        pa.out = static_cast<double*>(r.out);
+        pa.coe = params;
+        pa.numCoe = p->numCoe;
+        pa.dir = static_cast<int>(p->dir);
+        pa.ldIn = r.ldIn;
+        pa.ldOut = r.ldOut;
+        pa.nx = r.nx;
+        pa.ny = r.ny;
+        pa.H = p->H;
+        pa.L = p->L;
+        pa.V = p->V;
+        pa.T = p->T;
+        pa.xWrap = p->periodic ? 1 : 0;
+        pa.rowWrap = r.rows.wrap ? r.rows.rowWrap : 0;
+        pa.rowMin = r.rows.rowMin;
+        pa.rowMax = r.rows.rowMax;
+        pa.wx0 = wx0;
+        pa.wx1 = wx1;
+        pa.wy0 = wy0;
+        pa.wy1 = wy1;
+        pa.yGlobal0 = r.yGlobal0;
+        return p->pointLauncher(&pa, st);
+    }
     FunLaunch f{};
     f.in = static_cast<const double*>(r.in);
     f.out = static_cast<double*>(r.out);
@@ -1246,6 +1273,18 @@ void custenSetForceIndirect(cuSten_t* h, int force)
     Plan* p = planOf(h, "custenSetForceIndirect");
     if (!p) return;
     p->forceIndirect = force != 0;
+}
+
+void custenSetPointLauncher(cuSten_t* h, custenPointLauncher launcher)
+{
+    Plan* p = planOf(h, "custenSetPointLauncher");
+    if (!p) return;
+    if (!p->fun || p->weno) {
+        setError(CUSTEN_ERROR_INVALID_VALUE,
+                 "custenSetPointLauncher: the handle is not a point-function handle");
+        return;
+    }
+    p->pointLauncher = launcher;
 }
 
 double* custenGetBuiltinFunction(const char* name, int deviceNum)

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/cuSten.h"
+#include "../../include/custen_point.h"
 #include "kernels.h"
 
 namespace custen {
@@ -27,6 +28,7 @@ struct Plan {
     int numCoe = 0;
     Builtin builtin = Builtin::None;
     bool forceIndirect = false;
+    custenPointLauncher pointLauncher = nullptr;  // caller-compiled point kernel
     bool zeroRight = false;     // x-np: right R columns written as 0.0
     bool weno = false;          // WENO5 advection (cuStenCreate2DXYWENOADVp)
     bool wenoSingleSquares = true;  // smoothness squares in single precision (as powf)

@@ -259,6 +259,82 @@ __global__ __launch_bounds__(kThreads) void funKernel(FunLaunch a)
     }
 }
 
+// Point functions reached through a pointer (the caller's own device
+// functions).  The call cannot be inlined, and the callee reads its window
+// and coefficients through generic pointers: every read is a FLAT
+// instruction, and the texture-address unit that processes them is the
+// bottleneck (measured: busy 96 % of the kernel's time, about 21 cycles per
+// 64-lane 8-byte read; a 3 x 3 weighted sum issues 18 of them per point).
+// The kernel therefore spends as few of those units' cycles as it can on
+// its own loads:
+//   - one wave per workgroup walks a 128-column strip (two columns per
+//     lane) over a band of kCallRows rows; each staged row is one 16-byte
+//     load per lane plus one load for the halo columns, so staging costs
+//     about one address instruction per 64 points;
+//   - every lane calls for every row of the band (lanes past the grid or the
+//     write window compute on staged values and do not store), so the call
+//     sits in straight-line code with a full execution mask;
+//   - results stay in registers until the band's last call: a callee starts
+//     by waiting for all of the wave's outstanding memory operations, stores
+//     included.
+constexpr int kCallRows = 8;
+
+template <int DIR>
+__global__ __launch_bounds__(kLanes) void funCallKernel(FunLaunch a)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int H = a.H, V = a.V, L = a.L;
+    const int R = H - 1 - L;
+    const int LP = (L + 1) & ~1;
+    const int LD = ldsRowLength(H);
+    const int NR = kCallRows + V - 1;
+    double* tile = reinterpret_cast<double*>(smem);
+    double* coeL = tile + NR * LD;
+    const int lane = threadIdx.x;
+    const int nx = a.nx;
+    const int stripsX = (nx + kTW - 1) / kTW;
+    const int g = xcdRemap(blockIdx.x, gridDim.x);
+    const int x0 = (g % stripsX) * kTW;
+    const int y0 = (g / stripsX) * kCallRows;
+    if (y0 >= a.ny) return;
+
+    for (int k = lane; k < a.numCoe; k += kLanes) coeL[k] = a.coe[k];
+    const bool vecIn = a.vecIn && (x0 + kTW <= nx);
+    for (int r = 0; r < NR; r++) {
+        const int sy = mapRow(a.rows, y0 - a.T + r);
+        stageRow<double, Xform::Identity>(tile + r * LD, a.in + (int64_t)sy * a.ldIn, x0, nx, L,
+                                          R, LP, a.xWrap, vecIn, lane);
+    }
+    __syncthreads();
+
+    // window of column x0 + 2 lane + e, row r: top-left at r LD + base + e
+    int base = (LP - L) + 2 * lane;
+    if constexpr (DIR == 0) base += L;           // window centre (V == 1)
+    else if constexpr (DIR == 1) base += a.T * LD;  // window centre (H == 1)
+    const int rows = min(kCallRows, a.ny - y0);
+    const int gyg0 = a.yGlobal0 + y0;
+    double res[kCallRows][2];
+#pragma unroll
+    for (int r = 0; r < kCallRows; r++) {
+        res[r][0] = res[r][1] = 0.0;
+        if (r < rows && gyg0 + r >= a.wy0 && gyg0 + r < a.wy1) {
+            res[r][0] = CallIndirect::call<DIR>(a.func, tile, coeL, base + r * LD, LD, H, V);
+            res[r][1] = CallIndirect::call<DIR>(a.func, tile, coeL, base + r * LD + 1, LD, H, V);
+        }
+    }
+    const int gx = x0 + 2 * lane;
+    double* orow = a.out + (int64_t)y0 * a.ldOut + gx;
+#pragma unroll
+    for (int r = 0; r < kCallRows; r++) {
+        if (r < rows && gyg0 + r >= a.wy0 && gyg0 + r < a.wy1) {
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+                if (gx + e < nx && gx + e >= a.wx0 && gx + e < a.wx1)
+                    orow[(int64_t)r * a.ldOut + e] = res[r][e];
+        }
+    }
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -618,10 +694,7 @@ hipError_t launchFun(const FunLaunch& in, hipStream_t st)
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
         const void* kernels[] = {reinterpret_cast<const void*>(&dev::funKernel<0, dev::CallXCentralDiff>),
-                                 reinterpret_cast<const void*>(&dev::funKernel<1, dev::CallYCentralDiff9>),
-                                 reinterpret_cast<const void*>(&dev::funKernel<0, dev::CallIndirect>),
-                                 reinterpret_cast<const void*>(&dev::funKernel<1, dev::CallIndirect>),
-                                 reinterpret_cast<const void*>(&dev::funKernel<2, dev::CallIndirect>)};
+                                 reinterpret_cast<const void*>(&dev::funKernel<1, dev::CallYCentralDiff9>)};
         for (const void* k : kernels) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
@@ -639,18 +712,36 @@ hipError_t launchFun(const FunLaunch& in, hipStream_t st)
         return hipGetLastError();
     }
     if (f.func == nullptr) return hipErrorInvalidValue;
+    // caller's function through its pointer: one wave per 128 x kCallRows block
+    const int64_t stripsX = (f.nx + dev::kTW - 1) / dev::kTW;
+    const int64_t waves = stripsX * ((f.ny + dev::kCallRows - 1) / dev::kCallRows);
+    if (waves > 0x7fffffff) return hipErrorInvalidValue;
+    const size_t ldsCall =
+        sizeof(double) * ((size_t)(dev::kCallRows + f.V - 1) * dev::ldsRowLength(f.H) +
+                          (size_t)(f.numCoe > 0 ? f.numCoe : 1));
+    if (ldsCall > 160 * 1024) return hipErrorInvalidValue;
+    if (ldsCall > 64 * 1024) {
+        const void* kernels[] = {reinterpret_cast<const void*>(&dev::funCallKernel<0>),
+                                 reinterpret_cast<const void*>(&dev::funCallKernel<1>),
+                                 reinterpret_cast<const void*>(&dev::funCallKernel<2>)};
+        for (const void* k : kernels) {
+            hipError_t e =
+                hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsCall);
+            if (e != hipSuccess) return e;
+        }
+    }
     switch (f.dir) {
     case 0:
-        hipLaunchKernelGGL((dev::funKernel<0, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
-                           lds, st, f);
+        hipLaunchKernelGGL((dev::funCallKernel<0>), dim3((unsigned)waves), dim3(dev::kLanes),
+                           ldsCall, st, f);
         break;
     case 1:
-        hipLaunchKernelGGL((dev::funKernel<1, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
-                           lds, st, f);
+        hipLaunchKernelGGL((dev::funCallKernel<1>), dim3((unsigned)waves), dim3(dev::kLanes),
+                           ldsCall, st, f);
         break;
     default:
-        hipLaunchKernelGGL((dev::funKernel<2, dev::CallIndirect>), dim3(grid), dim3(dev::kThreads),
-                           lds, st, f);
+        hipLaunchKernelGGL((dev::funCallKernel<2>), dim3((unsigned)waves), dim3(dev::kLanes),
+                           ldsCall, st, f);
         break;
     }
     return hipGetLastError();

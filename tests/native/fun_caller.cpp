@@ -20,9 +20,14 @@
 //     1. x periodic,  5-point weighted sum  (devArg1X,  xWeightedSum5)
 //     2. xy periodic, 3x3 weighted sum      (devArg1XY, centralDiff as in 2d_xy_p_fun.cu)
 //     3. xy non-periodic, 3x3 sum of coe * (u^3 - u)  (devArg1XY, nonLinRHS)
+// usage: fun_caller time <n> <reps>
+//   times `reps` xy-periodic applies of each of this file's xy functions on
+//   an n x n device-resident field (HIP events, after warm-up applies) and
+//   prints one JSON line with the microseconds per apply
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include <hip/hip_runtime.h>
 
@@ -82,8 +87,55 @@ void check(hipError_t e, const char* what)
 
 }  // namespace
 
+int timeApplies(int n, int reps)
+{
+    const size_t pts = (size_t)n * n;
+    double *in = nullptr, *out = nullptr, *coe = nullptr;
+    check(hipMalloc(&in, pts * sizeof(double)), "hipMalloc");
+    check(hipMalloc(&out, pts * sizeof(double)), "hipMalloc");
+    check(hipMalloc(&coe, 9 * sizeof(double)), "hipMalloc");
+    double* host = static_cast<double*>(std::malloc(pts * sizeof(double)));
+    custenFillUniform(host, pts, 2, -1.0, 1.0);
+    check(hipMemcpy(in, host, pts * sizeof(double), hipMemcpyHostToDevice), "upload");
+    std::free(host);
+    double hc[9];
+    custenFillUniform(hc, 9, 3, -1.0, 1.0);
+    check(hipMemcpy(coe, hc, sizeof(hc), hipMemcpyHostToDevice), "upload");
+    double* funcs[2] = {nullptr, nullptr};
+    check(hipMemcpyFromSymbol(&funcs[0], HIP_SYMBOL(devFuncNonLin), sizeof(devArg1XY)), "fetch");
+    check(hipMemcpyFromSymbol(&funcs[1], HIP_SYMBOL(devFuncXY), sizeof(devArg1XY)), "fetch");
+    const char* names[2] = {"nonLinRHS", "centralDiff"};
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    std::printf("{\"n\": %d, \"reps\": %d", n, reps);
+    for (int k = 0; k < 2; k++) {
+        cuSten_t h;
+        cuStenCreate2DXYpFun(&h, 0, 1, n, n, 32, 32, out, in, coe, 3, 1, 1, 3, 1, 1, funcs[k]);
+        checkError("create");
+        for (int i = 0; i < 5; i++) cuStenCompute2DXYpFun(&h, DEVICE);
+        check(hipDeviceSynchronize(), "sync");
+        check(hipEventRecord(e0, h.streams[0]), "record");
+        for (int i = 0; i < reps; i++) cuStenCompute2DXYpFun(&h, DEVICE);
+        check(hipEventRecord(e1, h.streams[0]), "record");
+        check(hipEventSynchronize(e1), "sync");
+        checkError("compute");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        std::printf(", \"%s_us\": %.1f", names[k], 1e3 * ms / reps);
+        cuStenDestroy2DXYpFun(&h);
+    }
+    std::printf("}\n");
+    (void)hipFree(in);
+    (void)hipFree(out);
+    (void)hipFree(coe);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 4 && std::string(argv[1]) == "time")
+        return timeApplies(std::atoi(argv[2]), std::atoi(argv[3]));
     if (argc != 5) {
         std::fprintf(stderr, "usage: %s nx ny seed out.bin\n", argv[0]);
         return 2;

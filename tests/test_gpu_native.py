@@ -6,7 +6,10 @@
   fun_caller      a C++ main with its own __device__ point functions, fetched
                   with hipMemcpyFromSymbol as in examples/src/2d_xy_p_fun.cu
                   and cuPentCahnADI.cu:164-190,486-516, compiled with
-                  -fgpu-rdc and linked against libcusten.a.
+                  -fgpu-rdc and linked against libcusten.a;
+  point_caller    the same kind of caller with its functions compiled into
+                  its own kernels (include/custen_point.h), linked against
+                  libcusten.so without relocatable device code.
 
 What each writes is compared bit for bit with the oracle."""
 import os
@@ -74,3 +77,21 @@ def test_fun_caller_own_device_functions(gpu, tmp_path):
     assert mismatches(got[0], ref_x) == 0, "x periodic, caller's devArg1X"
     assert mismatches(got[1], ref_xy) == 0, "xy periodic, caller's devArg1XY"
     assert mismatches(got[2], ref_nl) == 0, "xy non-periodic, caller's nonLinRHS"
+
+
+def test_point_caller_inlined_functions(gpu, tmp_path):
+    nx, ny, seed = 520, 136, 6300
+    out = tmp_path / "point.bin"
+    run(["point_caller", nx, ny, seed, out], tmp_path)
+    got = np.fromfile(out, dtype=np.float64).reshape(5, ny, nx)
+    u = synth.field(nx, ny, seed)
+    coe = synth.uniform(9, seed + 1, -1.0, 1.0)
+    ref_x = oracle.apply_fun(oracle.FUN_X_WSUM, 0, u, coe[:5], 5, 2, 1, 0, True)
+    ref_y = oracle.apply_fun(oracle.FUN_Y_WSUM, 1, u, coe[:5], 1, 0, 5, 2, False)
+    ref_xy = oracle.apply_fun(oracle.FUN_XY_WSUM, 2, u, coe, 3, 1, 3, 1, True)
+    ref_nl = oracle.apply_fun(oracle.FUN_XY_CUBE_MINUS, 2, u, coe, 3, 1, 3, 1, False)
+    assert mismatches(got[0], ref_x) == 0, "x periodic"
+    assert mismatches(got[1], ref_y) == 0, "y non-periodic"
+    assert mismatches(got[2], ref_xy) == 0, "xy periodic"
+    assert mismatches(got[3], ref_nl) == 0, "xy non-periodic nonLinRHS"
+    assert mismatches(got[4], ref_xy) == 0, "xy periodic, host-streamed in 4 tiles"
