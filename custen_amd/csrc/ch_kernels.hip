@@ -164,73 +164,54 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// column solve: one thread per column (coalesced across the wave), rows
-// streamed with a block of loads in flight ahead of the recurrence.
+// column solve: one thread per column (coalesced across the wave).
 // pentSolve (serialCahnADI.c:161-241) + solveEnd (:628-655); the column's
 // solveFull is applied by the next phase.  which = 0: Y, which = 1: X.
+//
+// A sweep advances in steps of SR rows.  A wave keeps three steps of its
+// column in registers: the step it computes and the two after it, whose
+// loads were issued one and two steps earlier; the buffers rotate by
+// unrolling three steps, so no register copy waits on a load in flight.  The
+// diagonals (shared by every column of a request) of the next step are
+// loaded during the current one and written to a two-slot LDS ring at its
+// end; rows are read from the ring as broadcasts.  Inside the unrolled loop
+// every load and store is unconditional (row indices past the sweep are
+// clamped to a valid row and their values unused), so the wait counts the
+// compiler computes are exact and no wait drains the loads still in flight.
+// Steps that do not fit whole are done one row at a time after the loop.
 // ---------------------------------------------------------------------------
-constexpr int PF = 16;  // rows of the column in registers ahead of the recurrence
-constexpr int DB = 64;  // rows of the shared diagonals staged in LDS per block
+constexpr int SR = 16;  // rows per step
 
-// One PF-row block of the forward sweep (rows i0 .. i0+PF-1, guarded against
-// m when !FULL): diagonals from LDS into registers first, then the recurrence
-// x_i = ((b_i - ds_i x_{i-2}) - dl_i x_{i-1}) / d_i.
-template <bool FULL>
-__device__ __forceinline__ void forwardBlock(gdouble* __restrict__ base, int64_t s, int col, int i0,
-                                             int m, const double (*dg)[DB], int o,
-                                             const double (&cur)[PF], double& b2, double& b1)
+// Reciprocal of a diagonal entry for the division below: RN(1/d) when
+// 2^-60 <= |d| <= 2^60, else 0 (the row then divides with IEEE division).
+__device__ __forceinline__ double divRecip(double d)
 {
-    double a[PF], l[PF], d[PF];
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-        a[k] = dg[0][o + k];
-        l[k] = dg[1][o + k];
-        d[k] = dg[2][o + k];
-    }
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-        if (FULL || i0 + k < m) {
-            const double t = cur[k] - a[k] * b2;
-            const double v = (t - l[k] * b1) / d[k];
-            base[(int64_t)(i0 + k) * s + col] = v;
-            b2 = b1;
-            b1 = v;
-        }
-    }
+    const double ad = __builtin_fabs(d);
+    return (ad >= 0x1p-60 && ad <= 0x1p60) ? 1.0 / d : 0.0;
 }
 
-// One PF-row block of the backward sweep (rows i0, i0-1, ..., i0-PF+1):
-// x_i = (y_i - du_i x_{i+1}) - dw_i x_{i+2}.
-template <bool FULL>
-__device__ __forceinline__ void backwardBlock(gdouble* __restrict__ base, int64_t s, int col, int i0,
-                                              const double (*dg)[DB], int o,
-                                              const double (&cur)[PF], double& a0, double& a1)
+// n / d, correctly rounded, with y = RN(1/d) precomputed (all columns of a
+// request share the matrix, so y is computed once per row, off the
+// recurrence's critical path).  q0 = RN(n y) and two corrections
+// q <- RN(q + RN(n - d q) y) (each remainder exact by fma) give RN(n / d)
+// (Markstein's theorem: the first correction makes q faithful, the second
+// rounds it correctly); the dependent chain is five operations against the
+// eleven of the scaled IEEE sequence.  The preconditions (no overflow and no
+// subnormal intermediate) hold for 2^-900 <= |n| <= 2^900 and the range of d
+// in divRecip (y != 0).
+__device__ __forceinline__ double divByRecip(double n, double d, double y)
 {
-    double u[PF], w[PF];
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-        u[k] = dg[0][o + k];
-        w[k] = dg[1][o + k];
-    }
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-        if (FULL || i0 - k >= 0) {
-            const double t = cur[k] - u[k] * a0;
-            const double v = t - w[k] * a1;
-            base[(int64_t)(i0 - k) * s + col] = v;
-            a1 = a0;
-            a0 = v;
-        }
-    }
+    double q = n * y;
+    q = __builtin_fma(__builtin_fma(-d, q, n), y, q);
+    return __builtin_fma(__builtin_fma(-d, q, n), y, q);
 }
 
 __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
                                                      int which)
 {
-    // Diagonals of the current and the next DB-row block (every column of a
-    // request shares one matrix): staged in LDS one block ahead, read as
-    // broadcasts.  Forward uses ds, dl, d; backward du, dw.
-    __shared__ double sd[2][3][DB];
+    // ring of diagonals: [slot][array][row of the step]; every lane writes
+    // its own row of a slot (rows SR..63 are not read)
+    __shared__ double sd[2][4][kCHColBlock];
     const int item = blockIdx.x;
     const int r = findRequest(descs, count, item, 2);
     const CHDesc& D = descs[r];
@@ -248,127 +229,152 @@ __global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ 
     const gdouble* __restrict__ dd = dl + m;
     const gdouble* __restrict__ du = dd + m;
     const gdouble* __restrict__ dw = du + m;
+    auto at = [&](int row) -> gdouble& { return base[(int64_t)row * s + col]; };
 
-    // ---- forward substitution: rows 0, 1, then blocks of DB rows from row 2
-    double b2 = base[col] / dd[0];
-    base[col] = b2;
-    double b1 = (base[s + col] - dl[1] * b2) / dd[1];
-    base[s + col] = b1;
-    double p0 = 0.0, p1 = 0.0, p2 = 0.0;  // lane's rows of the next diagonal block
+    // ---- forward substitution: rows 0 and 1, then rows 2 .. m-1 in steps
+    double b2 = at(0) / dd[0];
+    at(0) = b2;
+    double b1 = (at(1) - dl[1] * b2) / dd[1];
+    at(1) = b1;
     {
-        const int rr = 2 + lane;
-        if (rr < m) {
-            sd[0][0][lane] = ds[rr];
-            sd[0][1][lane] = dl[rr];
-            sd[0][2][lane] = dd[rr];
-        }
-        const int rn = 2 + DB + lane;
-        if (rn < m) {
-            p0 = ds[rn];
-            p1 = dl[rn];
-            p2 = dd[rn];
-        }
-    }
-    __syncthreads();
-    double cur[PF], nxt[PF];
+        const int nFull = (m - 2) / SR;  // whole steps
+        auto row = [&](int j, int k) { return min(2 + j * SR + k, m - 1); };
+        auto loadStep = [&](double (&buf)[SR], int j) {
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (2 + k < m) ? base[(int64_t)(2 + k) * s + col] : 0.0;
-    for (int i0 = 2; i0 < m; i0 += PF) {
-        const int rel = i0 - 2;
-        const int blk = rel / DB;
-        if (rel > 0 && rel % DB == 0) {
-            // block blk starts: its diagonals (prefetched during block blk-1)
-            // go to LDS buffer blk & 1, then block blk+1 is prefetched
-            sd[blk & 1][0][lane] = p0;
-            sd[blk & 1][1][lane] = p1;
-            sd[blk & 1][2][lane] = p2;
-            const int rn = 2 + (blk + 1) * DB + lane;
-            if (rn < m) {
-                p0 = ds[rn];
-                p1 = dl[rn];
-                p2 = dd[rn];
+            for (int k = 0; k < SR; k++) buf[k] = at(row(j, k));
+        };
+        auto stageDiag = [&](int slot, double p0, double p1, double p2) {
+            sd[slot][0][lane] = p0;
+            sd[slot][1][lane] = p1;
+            sd[slot][2][lane] = p2;
+            sd[slot][3][lane] = divRecip(p2);
+        };
+        // step j: diagonals of step j+1 and the column rows of step j+2 are
+        // loaded, step j is computed from `cur` and ring slot j & 1
+        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+            const int rd = row(j + 1, lane < SR ? lane : SR - 1);
+            const double p0 = ds[rd], p1 = dl[rd], p2 = dd[rd];
+            loadStep(ahead, j + 2);
+            const int slot = j & 1;
+            double nmin = 0x1p900, nmax = 0x1p-900, ymin = 1.0;
+            const double b2s = b2, b1s = b1;
+#pragma unroll
+            for (int k = 0; k < SR; k++) {
+                const double y = sd[slot][3][k];
+                const double t = cur[k] - sd[slot][0][k] * b2;
+                const double num = t - sd[slot][1][k] * b1;
+                const double v = divByRecip(num, sd[slot][2][k], y);
+                nmin = __builtin_fmin(nmin, __builtin_fabs(num));
+                nmax = __builtin_fmax(nmax, __builtin_fabs(num));
+                ymin = __builtin_fmin(ymin, __builtin_fabs(y));
+                at(2 + j * SR + k) = v;
+                b2 = b1;
+                b1 = v;
             }
-            __syncthreads();
-        }
-        if (i0 + 2 * PF <= m) {
-#pragma unroll
-            for (int k = 0; k < PF; k++) nxt[k] = base[(int64_t)(i0 + PF + k) * s + col];
-        } else {
-#pragma unroll
-            for (int k = 0; k < PF; k++) {
-                const int i = i0 + PF + k;
-                nxt[k] = i < m ? base[(int64_t)i * s + col] : 0.0;
+            // a numerator outside divByRecip's range (zero, whose sign the
+            // corrections would lose, subnormal, huge, Inf) or a row without
+            // a reciprocal: this lane repeats the step with IEEE division
+            if (__builtin_expect(!(nmin >= 0x1p-900 && nmax <= 0x1p900 && ymin != 0.0), 0)) {
+                b2 = b2s;
+                b1 = b1s;
+                for (int k = 0; k < SR; k++) {
+                    const double t = cur[k] - sd[slot][0][k] * b2;
+                    const double v = (t - sd[slot][1][k] * b1) / sd[slot][2][k];
+                    at(2 + j * SR + k) = v;
+                    b2 = b1;
+                    b1 = v;
+                }
             }
+            stageDiag(slot ^ 1, p0, p1, p2);
+        };
+        double bufA[SR], bufB[SR], bufC[SR];
+        {
+            const int r0 = row(0, lane < SR ? lane : SR - 1);
+            stageDiag(0, ds[r0], dl[r0], dd[r0]);
         }
-        const double(*dg)[DB] = sd[blk & 1];
-        const int o = rel - blk * DB;
-        if (i0 + PF <= m) forwardBlock<true>(base, s, col, i0, m, dg, o, cur, b2, b1);
-        else forwardBlock<false>(base, s, col, i0, m, dg, o, cur, b2, b1);
-#pragma unroll
-        for (int k = 0; k < PF; k++) cur[k] = nxt[k];
+        loadStep(bufA, 0);
+        loadStep(bufB, 1);
+        int j = 0;
+        for (; j + 3 <= nFull; j += 3) {
+            step(j, bufA, bufC);
+            step(j + 1, bufB, bufA);
+            step(j + 2, bufC, bufB);
+        }
+        if (j < nFull) step(j++, bufA, bufC);
+        if (j < nFull) step(j++, bufB, bufA);
+        // rows left after the whole steps, one at a time
+        for (int i = 2 + nFull * SR; i < m; i++) {
+            const double t = at(i) - ds[i] * b2;
+            const double v = (t - dl[i] * b1) / dd[i];
+            at(i) = v;
+            b2 = b1;
+            b1 = v;
+        }
     }
 
-    // ---- backward substitution: b1 = x[m-1], b2 = b[m-2]; blocks of DB rows
-    // descending from row m-3
+    // ---- backward substitution: b1 = x[m-1], b2 = b[m-2]; then rows m-3
+    // down to 0 in steps
     const double xm1 = b1;
     double a1 = b1;
     double a0 = b2 - du[m - 2] * a1;
-    base[(int64_t)(m - 2) * s + col] = a0;
+    at(m - 2) = a0;
     const double xm2 = a0;
-    __syncthreads();  // every read of the forward blocks is done
     {
-        const int rr = m - 3 - lane;
-        if (rr >= 0) {
-            sd[0][0][lane] = du[rr];
-            sd[0][1][lane] = dw[rr];
-        }
-        const int rn = m - 3 - DB - lane;
-        p0 = p1 = 0.0;
-        if (rn >= 0) {
-            p0 = du[rn];
-            p1 = dw[rn];
-        }
-    }
-    __syncthreads();
+        const int nFull = (m - 2) / SR;
+        auto row = [&](int j, int k) { return max(m - 3 - j * SR - k, 0); };
+        auto loadStep = [&](double (&buf)[SR], int j) {
 #pragma unroll
-    for (int k = 0; k < PF; k++) cur[k] = (m - 3 - k >= 0) ? base[(int64_t)(m - 3 - k) * s + col] : 0.0;
-    for (int i0 = m - 3; i0 >= 0; i0 -= PF) {
-        const int rel = m - 3 - i0;
-        const int blk = rel / DB;
-        if (rel > 0 && rel % DB == 0) {
-            sd[blk & 1][0][lane] = p0;
-            sd[blk & 1][1][lane] = p1;
-            const int rn = m - 3 - (blk + 1) * DB - lane;
-            if (rn >= 0) {
-                p0 = du[rn];
-                p1 = dw[rn];
+            for (int k = 0; k < SR; k++) buf[k] = at(row(j, k));
+        };
+        auto stageDiag = [&](int slot, double p0, double p1) {
+            sd[slot][0][lane] = p0;
+            sd[slot][1][lane] = p1;
+        };
+        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+            const int rd = row(j + 1, lane < SR ? lane : SR - 1);
+            const double p0 = du[rd], p1 = dw[rd];
+            loadStep(ahead, j + 2);
+            const int slot = j & 1;
+#pragma unroll
+            for (int k = 0; k < SR; k++) {
+                const double t = cur[k] - sd[slot][0][k] * a0;
+                const double v = t - sd[slot][1][k] * a1;
+                at(m - 3 - j * SR - k) = v;
+                a1 = a0;
+                a0 = v;
             }
-            __syncthreads();
+            stageDiag(slot ^ 1, p0, p1);
+        };
+        double bufA[SR], bufB[SR], bufC[SR];
+        {
+            const int r0 = row(0, lane < SR ? lane : SR - 1);
+            stageDiag(0, du[r0], dw[r0]);
         }
-        if (i0 - 2 * PF + 1 >= 0) {
-#pragma unroll
-            for (int k = 0; k < PF; k++) nxt[k] = base[(int64_t)(i0 - PF - k) * s + col];
-        } else {
-#pragma unroll
-            for (int k = 0; k < PF; k++) {
-                const int i = i0 - PF - k;
-                nxt[k] = i >= 0 ? base[(int64_t)i * s + col] : 0.0;
-            }
+        loadStep(bufA, 0);
+        loadStep(bufB, 1);
+        int j = 0;
+        for (; j + 3 <= nFull; j += 3) {
+            step(j, bufA, bufC);
+            step(j + 1, bufB, bufA);
+            step(j + 2, bufC, bufB);
         }
-        const double(*dg)[DB] = sd[blk & 1];
-        const int o = rel - blk * DB;
-        if (i0 - PF + 1 >= 0) backwardBlock<true>(base, s, col, i0, dg, o, cur, a0, a1);
-        else backwardBlock<false>(base, s, col, i0, dg, o, cur, a0, a1);
-#pragma unroll
-        for (int k = 0; k < PF; k++) cur[k] = nxt[k];
+        if (j < nFull) step(j++, bufA, bufC);
+        if (j < nFull) step(j++, bufB, bufA);
+        for (int i = m - 3 - nFull * SR; i >= 0; i--) {
+            const double t = at(i) - du[i] * a0;
+            const double v = t - dw[i] * a1;
+            at(i) = v;
+            a1 = a0;
+            a0 = v;
+        }
     }
     if (!live) return;
     // solveEnd: a0 = x[0], a1 = x[1]
     gdouble* __restrict__ Xc = base + col;
-    const double nx2 = Xc[(n - 2) * s] - (D.e * a0 + D.a * xm2 + D.b * xm1);
-    const double nx1 = Xc[(n - 1) * s] - (D.d * a0 + D.e * a1 + D.a * xm1);
-    Xc[(n - 2) * s] = D.omega[0] * nx2 + D.omega[1] * nx1;
-    Xc[(n - 1) * s] = D.omega[2] * nx2 + D.omega[3] * nx1;
+    const double e2 = Xc[(n - 2) * s] - (D.e * a0 + D.a * xm2 + D.b * xm1);
+    const double e1 = Xc[(n - 1) * s] - (D.d * a0 + D.e * a1 + D.a * xm1);
+    Xc[(n - 2) * s] = D.omega[0] * e2 + D.omega[1] * e1;
+    Xc[(n - 1) * s] = D.omega[2] * e2 + D.omega[3] * e1;
 }
 
 // ---------------------------------------------------------------------------
