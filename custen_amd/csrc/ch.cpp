@@ -21,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cuSten.h"
@@ -167,10 +168,26 @@ struct CHRequest {
     CHBuffers buf;
     std::vector<double> c0;
     std::vector<double> cOld0;  // empty: cOld starts equal to c0
+    bool staged = false;        // buf already holds c0, cOld0 and the factors
     std::vector<double> result;
-    int state = 0;  // 0 queued, 1 resident, 2 finished
+    double* pinnedResult = nullptr;  // the result, when held in pinned memory
+    int state = 0;  // 0 queued, 1 resident (or its result in transit), 2 finished
     Clock::time_point tSubmit, tAdmit, tFinish;
 };
+
+// Writes a request's fields and factors into its device buffers: c0 into
+// f[0]; cOld0 into f[1], or a copy of c0 when cOld0 is null.
+hipError_t loadRequest(const CHRequest* r, const double* c0, const double* cOld0, hipStream_t st)
+{
+    const size_t bytes = (size_t)r->co.n * r->co.n * sizeof(double);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(r->buf.f[0], c0, bytes, hipMemcpyDefault, st)) != hipSuccess) return e;
+    if (cOld0) e = hipMemcpyAsync(r->buf.f[1], cOld0, bytes, hipMemcpyDefault, st);
+    else e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(r->buf.fac, r->co.fac.data(), r->co.fac.size() * sizeof(double),
+                          hipMemcpyHostToDevice, st);
+}
 
 // Advances every resident request by one step with one launch per phase.
 class CHEngine {
@@ -178,8 +195,9 @@ public:
     explicit CHEngine(int device) : device_(device) {}
     ~CHEngine()
     {
-        if (dDescs_) (void)hipFree(dDescs_);
-        if (hDescs_) (void)hipHostFree(hDescs_);
+        for (BatchGraph& g : graphs_)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        releaseDescs();
     }
 
     int step() const { return step_; }
@@ -188,19 +206,19 @@ public:
     // otherwise the pair (c0, cOld0) resumes a simulation exactly.
     hipError_t admit(CHRequest* r, const double* c0, const double* cOld0, hipStream_t st)
     {
-        const size_t bytes = (size_t)r->co.n * r->co.n * sizeof(double);
-        hipError_t e;
-        if ((e = hipMemcpyAsync(r->buf.f[0], c0, bytes, hipMemcpyDefault, st)) != hipSuccess) return e;
-        if (cOld0) e = hipMemcpyAsync(r->buf.f[1], cOld0, bytes, hipMemcpyDefault, st);
-        else e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st);
+        hipError_t e = loadRequest(r, c0, cOld0, st);
         if (e != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(r->buf.fac, r->co.fac.data(), r->co.fac.size() * sizeof(double),
-                                hipMemcpyHostToDevice, st)) != hipSuccess)
-            return e;
+        adopt(r);
+        return hipSuccess;
+    }
+
+    // A request whose buffers already hold its fields and factors (written
+    // by loadRequest, on a stream the host has synchronised with).
+    void adopt(CHRequest* r)
+    {
         r->admitStep = step_;
         active_.push_back(r);
         dirty_ = true;
-        return hipSuccess;
     }
 
     void remove(CHRequest* r)
@@ -251,30 +269,101 @@ public:
         return hipSuccess;
     }
 
+    // k steps of all resident requests as one launch: the step kernels of the
+    // k steps are captured once into a HIP graph and replayed while the
+    // resident set (and so the descriptors) and k stay the same.  The
+    // kernels use a step number only through its parity relative to each
+    // request's admission, so a graph serves every batch whose first step has
+    // the parity it was captured at.  Replaying one graph costs the host one
+    // launch instead of 6 k.  Consecutive batches alternate between two
+    // graphs: launching a graph whose previous launch is still running makes
+    // the host wait for it.
+    hipError_t stepBatch(hipStream_t st, int k)
+    {
+        if (active_.empty() || k <= 0) return hipSuccess;
+        hipError_t e;
+        if (dirty_ && (e = upload(st)) != hipSuccess) return e;
+        BatchGraph& g = graphs_[launches_ & 1];
+        if (!g.exec || g.k != k || g.version != version_ || g.parity != (step_ & 1)) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return e;
+            const int s0 = step_;
+            hipError_t le = hipSuccess;
+            for (int i = 0; i < k && le == hipSuccess; i++) le = launchStep(st, s0 + i);
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(st, &graph);
+            if (le != hipSuccess) e = le;
+            if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                g.exec = nullptr;
+                return e;
+            }
+            g.k = k;
+            g.version = version_;
+            g.parity = step_ & 1;
+        }
+        if ((e = hipGraphLaunch(g.exec, st)) != hipSuccess) return e;
+        launches_++;
+        step_ += k;
+        return hipSuccess;
+    }
+
 private:
+    void releaseDescs()
+    {
+        if (dDescs_) (void)hipFree(dDescs_);
+        dDescs_ = nullptr;
+        for (int i = 0; i < 2; i++) {
+            if (staged_[i]) (void)hipEventSynchronize(staged_[i]);
+            if (hDescs_[i]) (void)hipHostFree(hDescs_[i]);
+            if (staged_[i]) (void)hipEventDestroy(staged_[i]);
+            hDescs_[i] = nullptr;
+            staged_[i] = nullptr;
+            stagedUsed_[i] = false;
+        }
+    }
+
+    hipError_t launchStep(hipStream_t st, int step)
+    {
+        hipError_t e;
+        if ((e = chLaunchRhs(dDescs_, count_, tiles_, step, st)) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 0, st)) != hipSuccess) return e;
+        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 1, st)) != hipSuccess) return e;
+        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
+        return chLaunchFinal(dDescs_, count_, tiles_, step, st);
+    }
+
     hipError_t upload(hipStream_t st)
     {
         const int n = (int)active_.size();
+        hipError_t e;
         if (n > cap_) {
             // the previous descriptor upload may still be read by queued launches
-            hipError_t e = hipStreamSynchronize(st);
-            if (e != hipSuccess) return e;
-            if (dDescs_) (void)hipFree(dDescs_);
-            if (hDescs_) (void)hipHostFree(hDescs_);
-            dDescs_ = nullptr;
-            hDescs_ = nullptr;
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+            releaseDescs();
             cap_ = std::max(n, 2 * cap_);
             if ((e = hipMalloc(&dDescs_, sizeof(CHDesc) * cap_)) != hipSuccess) return e;
-            if ((e = hipHostMalloc(&hDescs_, sizeof(CHDesc) * cap_, 0)) != hipSuccess) return e;
-        } else {
-            // pinned staging is reused: wait until the previous copy consumed it
-            hipError_t e = hipStreamSynchronize(st);
-            if (e != hipSuccess) return e;
+            for (int i = 0; i < 2; i++) {
+                if ((e = hipHostMalloc(&hDescs_[i], sizeof(CHDesc) * cap_, 0)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&staged_[i], hipEventDisableTiming)) != hipSuccess)
+                    return e;
+                stagedUsed_[i] = false;
+            }
         }
+        // Two pinned staging copies used in turn; one is rewritten only after
+        // the copy that last read it has run.  The device array needs no such
+        // wait: the copy into it is queued behind every launch that reads it.
+        const int u = (int)(uploads_++ & 1);
+        if (stagedUsed_[u] && (e = hipEventSynchronize(staged_[u])) != hipSuccess) return e;
+        CHDesc* h = hDescs_[u];
         int tiles = 0, trs = 0, cols = 0;
         for (int i = 0; i < n; i++) {
             const CHRequest* r = active_[i];
-            CHDesc& d = hDescs_[i];
+            CHDesc& d = h[i];
             std::memset(&d, 0, sizeof(d));
             d.f[0] = r->buf.f[0];
             d.f[1] = r->buf.f[1];
@@ -297,13 +386,17 @@ private:
             trs += chTrTiles(d.n);
             cols += chColBlocks(d.n);
         }
-        hipError_t e = hipMemcpyAsync(dDescs_, hDescs_, sizeof(CHDesc) * n, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(dDescs_, h, sizeof(CHDesc) * n, hipMemcpyHostToDevice, st)) !=
+                hipSuccess ||
+            (e = hipEventRecord(staged_[u], st)) != hipSuccess)
+            return e;
+        stagedUsed_[u] = true;
         count_ = n;
         tiles_ = tiles;
         trTiles_ = trs;
         colBlocks_ = cols;
         dirty_ = false;
+        version_++;
         return hipSuccess;
     }
 
@@ -312,9 +405,22 @@ private:
     std::vector<CHRequest*> active_;
     bool dirty_ = true;
     CHDesc* dDescs_ = nullptr;
-    CHDesc* hDescs_ = nullptr;
+    CHDesc* hDescs_[2] = {nullptr, nullptr};  // pinned staging, used in turn
+    hipEvent_t staged_[2] = {nullptr, nullptr};
+    bool stagedUsed_[2] = {false, false};
+    uint64_t uploads_ = 0;
     int cap_ = 0;
     int count_ = 0, tiles_ = 0, trTiles_ = 0, colBlocks_ = 0;
+    // captured batches, used alternately
+    struct BatchGraph {
+        hipGraphExec_t exec = nullptr;
+        int k = 0;
+        int parity = 0;  // of the batch's first step
+        uint64_t version = 0;
+    };
+    BatchGraph graphs_[2];
+    uint64_t launches_ = 0;
+    uint64_t version_ = 0;  // bumped by every descriptor upload
 };
 
 }  // namespace
@@ -700,15 +806,55 @@ extern "C" size_t custenCHFootprint(int n) { return n >= 8 ? footprintBytes(n) :
 // ===========================================================================
 // Requests of one grid size share a lane: one batched engine on its own
 // stream.  Lanes advance independently (a lane of small grids is not held to
-// the pace of a lane of large ones), several steps per enqueue; the host only
-// waits when no lane can take more work.
+// the pace of a lane of large ones), several steps per enqueue, with up to
+// two batches in flight so that a lane keeps the GPU busy while the host
+// attends to the others; the host only waits when no lane can take more
+// work, and then for whichever batch finishes first.
 struct CHLane {
     int n = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t done[2] = {nullptr, nullptr};  // end of each batch in flight
+    int steps[2] = {0, 0};                     // steps of each batch in flight
+    int head = 0;                              // oldest batch in flight
+    int count = 0;                             // batches in flight
     std::unique_ptr<CHEngine> engine;
-    int inFlight = 0;            // steps enqueued and not yet accounted
-    Clock::time_point enqueued;  // when the in-flight batch was enqueued
+    int inFlight() const
+    {
+        int k = 0;
+        for (int i = 0; i < count; i++) k += steps[(head + i) & 1];
+        return k;
+    }
+};
+
+// Pinned host buffers for results, reused by size.
+class PinnedPool {
+public:
+    ~PinnedPool()
+    {
+        for (auto& kv : free_)
+            for (double* p : kv.second) (void)hipHostFree(p);
+    }
+    hipError_t get(size_t count, double*& out)
+    {
+        auto it = free_.find(count);
+        if (it != free_.end() && !it->second.empty()) {
+            out = it->second.back();
+            it->second.pop_back();
+            return hipSuccess;
+        }
+        return hipHostMalloc(&out, count * sizeof(double), 0);
+    }
+    void put(double* p, size_t count) { free_[count].push_back(p); }
+
+private:
+    std::map<size_t, std::vector<double*>> free_;
+};
+
+// Results copied out of a lane's buffers by one retirement, and the event
+// that marks the end of those copies.
+struct CHDrain {
+    hipEvent_t done = nullptr;
+    std::vector<CHRequest*> reqs;
 };
 
 struct custenCHServer {
@@ -716,10 +862,16 @@ struct custenCHServer {
     size_t budget = 0;
     int maxSlots = 0;
     int resident = 0;
-    size_t used = 0;
+    size_t used = 0;      // device bytes of staged, resident and draining requests
+    int unstaged = 0;     // queued requests whose input waits in host memory
+    int draining = 0;     // requests whose result is being copied out
+    size_t pinnedBytes = 0;  // results held in pinned memory
+    hipStream_t upload = nullptr;
     std::map<int, std::unique_ptr<CHLane>> lanes;
     std::unique_ptr<BufferPool> pool;
+    PinnedPool pinned;
     std::deque<CHRequest*> queue;
+    std::vector<CHDrain> drains;
     std::vector<std::unique_ptr<CHRequest>> all;
 };
 
@@ -727,14 +879,19 @@ namespace {
 
 // Steps a lane enqueues at once (fewer if a request finishes earlier).
 constexpr int kLaneBatch = 8;
+// Finished results kept in pinned memory up to this many bytes; later ones
+// move to pageable memory once their copy has ended.
+constexpr size_t kPinnedResultBytes = size_t(2) << 30;
 
 void releaseLane(CHLane& l)
 {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     l.engine.reset();
-    if (l.done) (void)hipEventDestroy(l.done);
+    for (hipEvent_t& ev : l.done) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
     if (l.stream) (void)hipStreamDestroy(l.stream);
-    l.done = nullptr;
     l.stream = nullptr;
 }
 
@@ -749,9 +906,11 @@ hipError_t laneFor(custenCHServer* s, int n, CHLane*& out)
     l->n = n;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking)) != hipSuccess) return e;
-    if ((e = hipEventCreateWithFlags(&l->done, hipEventDisableTiming)) != hipSuccess) {
-        releaseLane(*l);
-        return e;
+    for (hipEvent_t& ev : l->done) {
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            releaseLane(*l);
+            return e;
+        }
     }
     l->engine.reset(new CHEngine(s->device));
     out = l.get();
@@ -777,6 +936,10 @@ extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetB
     s->budget = hbmBudgetBytes;
     s->maxSlots = maxSlots;
     s->pool.reset(new BufferPool(hbmBudgetBytes / 4));
+    if ((e = hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking)) != hipSuccess) {
+        setHipError(e, "custenCHServerCreate");
+        return nullptr;
+    }
     return s.release();
 }
 
@@ -801,10 +964,34 @@ extern "C" int custenCHServerSubmitResume(custenCHServer* s, const custenCHParam
     r->id = (int)s->all.size();
     r->params = *params;
     r->steps = steps;
-    const size_t nn = (size_t)params->n * params->n;
-    r->c0.assign(c0, c0 + nn);
-    if (cOld0) r->cOld0.assign(cOld0, cOld0 + nn);
     r->tSubmit = Clock::now();
+    // Stage the input in HBM now when the budget has room and every request
+    // ahead of this one is staged too (so staged requests always head the
+    // queue and cannot hold memory that an earlier request needs); else keep
+    // a host copy until admission.
+    const size_t fp = footprintBytes(params->n);
+    (void)hipSetDevice(s->device);
+    bool stage = s->unstaged == 0 && s->used + fp <= s->budget;
+    if (stage && s->pool->get(params->n, r->buf) != hipSuccess) {
+        (void)hipGetLastError();  // a failed allocation only means "not now"
+        stage = false;
+    }
+    if (stage) {
+        hipError_t e = loadRequest(r.get(), c0, cOld0, s->upload);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->upload);
+        if (e != hipSuccess) {
+            s->pool->put(r->buf);
+            setHipError(e, "custenCHServerSubmit");
+            return -1;
+        }
+        r->staged = true;
+        s->used += fp;
+    } else {
+        const size_t nn = (size_t)params->n * params->n;
+        r->c0.assign(c0, c0 + nn);
+        if (cOld0) r->cOld0.assign(cOld0, cOld0 + nn);
+        s->unstaged++;
+    }
     s->queue.push_back(r.get());
     s->all.push_back(std::move(r));
     return s->all.back()->id;
@@ -818,136 +1005,224 @@ extern "C" int custenCHServerSubmit(custenCHServer* s, const custenCHParams* par
 
 namespace {
 
-// Account a lane's finished batch: every resident request advanced by the
-// batch's steps; finished ones are copied out and retired.
-hipError_t retireLane(custenCHServer* s, CHLane& l)
+// A request whose result has reached host memory: its device buffers go back
+// to the pool.
+void finishRequest(custenCHServer* s, CHRequest* r)
+{
+    s->pool->put(r->buf);
+    r->buf = CHBuffers{};
+    s->used -= footprintBytes(r->co.n);
+    std::vector<double>().swap(r->c0);
+    std::vector<double>().swap(r->cOld0);
+    r->tFinish = Clock::now();
+    r->state = 2;
+}
+
+// Account a lane's oldest batch, which has finished: every resident request
+// advanced by the batch's steps; finished ones leave the batch and their
+// results are copied to pinned host memory behind it on the lane's stream,
+// ending in a drain event (a batch never runs past a request's last step,
+// and a lane with a request about to finish has one batch in flight, so the
+// copies follow the request's last step and precede any later batch).
+hipError_t retireBatch(custenCHServer* s, CHLane& l)
 {
     hipError_t e;
+    const int k = l.steps[l.head];
+    l.steps[l.head] = 0;
+    l.head ^= 1;
+    l.count--;
     std::vector<CHRequest*> fin;
     for (CHRequest* r : l.engine->active()) {
-        r->done += l.inFlight;
+        r->done += k;
         if (r->done >= r->steps) fin.push_back(r);
     }
-    l.inFlight = 0;
+    if (fin.empty()) return hipSuccess;
+    CHDrain d;
     for (CHRequest* r : fin) {
         const size_t nn = (size_t)r->co.n * r->co.n;
-        r->result.resize(nn);
-        if ((e = hipMemcpyAsync(r->result.data(), l.engine->current(r), nn * sizeof(double),
-                                hipMemcpyDeviceToHost, l.stream)) != hipSuccess ||
-            (e = hipStreamSynchronize(l.stream)) != hipSuccess)
+        if ((e = s->pinned.get(nn, r->pinnedResult)) != hipSuccess) {
+            // no pinned memory: copy to pageable memory and wait
+            (void)hipGetLastError();
+            r->pinnedResult = nullptr;
+            r->result.resize(nn);
+            if ((e = hipMemcpyAsync(r->result.data(), l.engine->current(r), nn * sizeof(double),
+                                    hipMemcpyDeviceToHost, l.stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(l.stream)) != hipSuccess)
+                return e;
+            l.engine->remove(r);
+            s->resident--;
+            finishRequest(s, r);
+            continue;
+        }
+        s->pinnedBytes += nn * sizeof(double);
+        if ((e = hipMemcpyAsync(r->pinnedResult, l.engine->current(r), nn * sizeof(double),
+                                hipMemcpyDeviceToHost, l.stream)) != hipSuccess)
             return e;
-        r->tFinish = Clock::now();
-        r->state = 2;
         l.engine->remove(r);
-        s->pool->put(r->buf);
-        r->buf = CHBuffers{};
-        s->used -= footprintBytes(r->co.n);
         s->resident--;
-        std::vector<double>().swap(r->c0);
-        std::vector<double>().swap(r->cOld0);
+        s->draining++;
+        d.reqs.push_back(r);
     }
+    if (d.reqs.empty()) return hipSuccess;
+    if ((e = hipEventCreateWithFlags(&d.done, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventRecord(d.done, l.stream)) != hipSuccess)
+        return e;
+    s->drains.push_back(std::move(d));
     return hipSuccess;
 }
 
-// One scheduling round: retire the work of every lane whose batch finished,
-// admit queued requests (FIFO, while a slot and the HBM budget allow and the
-// request's lane is idle), enqueue the next batch of steps on every idle lane
-// with resident requests.  If no lane was idle, first wait for the batch that
-// was enqueued earliest.
+// Finish the requests of every drain whose copies have ended.  Returns the
+// number of requests finished, or -1 on error.
+int finishDrains(custenCHServer* s)
+{
+    int finished = 0;
+    for (size_t i = 0; i < s->drains.size();) {
+        CHDrain& d = s->drains[i];
+        const hipError_t q = hipEventQuery(d.done);
+        if (q == hipErrorNotReady) {
+            i++;
+            continue;
+        }
+        if (q != hipSuccess) {
+            setHipError(q, "custenCHServerStep (result copy)");
+            return -1;
+        }
+        for (CHRequest* r : d.reqs) {
+            finishRequest(s, r);
+            s->draining--;
+            finished++;
+            if (s->pinnedBytes > kPinnedResultBytes) {
+                const size_t nn = (size_t)r->co.n * r->co.n;
+                r->result.assign(r->pinnedResult, r->pinnedResult + nn);
+                s->pinned.put(r->pinnedResult, nn);
+                r->pinnedResult = nullptr;
+                s->pinnedBytes -= nn * sizeof(double);
+            }
+        }
+        (void)hipEventDestroy(d.done);
+        s->drains.erase(s->drains.begin() + i);
+    }
+    return finished;
+}
+
+// Retire every finished batch of every lane, and finish every ended drain,
+// without waiting.  Returns the number retired, or -1 on error.
+int retireFinished(custenCHServer* s)
+{
+    int retired = finishDrains(s);
+    if (retired < 0) return -1;
+    for (auto& kv : s->lanes) {
+        CHLane& l = *kv.second;
+        while (l.count > 0) {
+            const hipError_t q = hipEventQuery(l.done[l.head]);
+            if (q == hipErrorNotReady) break;
+            hipError_t e = q;
+            if (e == hipSuccess) e = retireBatch(s, l);
+            if (e != hipSuccess) {
+                setHipError(e, "custenCHServerStep (retire)");
+                return -1;
+            }
+            retired++;
+        }
+    }
+    return retired;
+}
+
+// One scheduling round: retire the lanes' finished batches, admit queued
+// requests (FIFO, while a slot and the HBM budget allow and the request's
+// lane is idle), and enqueue the next batch of steps on every lane with
+// resident requests and fewer than two batches in flight.  If nothing could
+// be done without waiting, first wait for whichever batch in flight finishes
+// first.
 int serverIteration(custenCHServer* s)
 {
     (void)hipSetDevice(s->device);
     hipError_t e;
-    auto idle = [](CHLane& l) {
-        if (l.inFlight == 0) return true;
-        const hipError_t q = hipEventQuery(l.done);
-        if (q == hipErrorNotReady) return false;
-        return true;  // finished, or an error that the next call reports
-    };
-    // can this round do anything without waiting?
-    bool canWork = false;
-    bool anyInFlight = false;
-    for (auto& kv : s->lanes) {
-        CHLane& l = *kv.second;
-        anyInFlight = anyInFlight || l.inFlight > 0;
-        if (l.inFlight > 0 ? idle(l) : !l.engine->active().empty()) canWork = true;
-    }
-    if (!canWork && !s->queue.empty() && s->resident < s->maxSlots) {
-        const CHRequest* r = s->queue.front();
-        auto it = s->lanes.find(r->co.n);
-        if (s->used + footprintBytes(r->co.n) <= s->budget &&
-            (it == s->lanes.end() || it->second->inFlight == 0))
-            canWork = true;
-    }
-    if (!canWork && anyInFlight) {
-        CHLane* first = nullptr;
-        for (auto& kv : s->lanes)
-            if (kv.second->inFlight > 0 && (!first || kv.second->enqueued < first->enqueued))
-                first = kv.second.get();
-        if (first && (e = hipEventSynchronize(first->done)) != hipSuccess) {
-            setHipError(e, "custenCHServerStep");
-            return -1;
-        }
-    }
-    for (auto& kv : s->lanes) {
-        CHLane& l = *kv.second;
-        if (l.inFlight == 0 || !idle(l)) continue;
-        if ((e = hipEventSynchronize(l.done)) != hipSuccess ||
-            (e = retireLane(s, l)) != hipSuccess) {
-            setHipError(e, "custenCHServerStep (retire)");
-            return -1;
-        }
-    }
-    // admission: FIFO while a slot and the HBM budget allow
-    while (!s->queue.empty() && s->resident < s->maxSlots) {
-        CHRequest* r = s->queue.front();
-        const size_t fp = footprintBytes(r->co.n);
-        if (s->used + fp > s->budget) break;
-        CHLane* l = nullptr;
-        if ((e = laneFor(s, r->co.n, l)) != hipSuccess) {
-            setHipError(e, "custenCHServerStep (lane)");
-            return -1;
-        }
-        if (l->inFlight > 0) break;  // its lane is mid-batch: admit at the next round
-        if ((e = s->pool->get(r->co.n, r->buf)) != hipSuccess) {
-            if (s->resident == 0) {
-                setHipError(e, "custenCHServerStep (allocation)");
+    for (int attempt = 0;; attempt++) {
+        const int retired = retireFinished(s);
+        if (retired < 0) return -1;
+        // admission: FIFO while a slot and the HBM budget allow
+        int admitted = 0;
+        while (!s->queue.empty() && s->resident < s->maxSlots) {
+            CHRequest* r = s->queue.front();
+            const size_t fp = footprintBytes(r->co.n);
+            if (!r->staged && s->used + fp > s->budget) break;
+            CHLane* l = nullptr;
+            if ((e = laneFor(s, r->co.n, l)) != hipSuccess) {
+                setHipError(e, "custenCHServerStep (lane)");
                 return -1;
             }
-            break;  // wait for memory to free up
+            if (l->count > 0) break;  // its lane is mid-batch: admit at a later round
+            if (r->staged) {
+                l->engine->adopt(r);
+            } else {
+                if ((e = s->pool->get(r->co.n, r->buf)) != hipSuccess) {
+                    if (s->resident == 0 && s->draining == 0) {
+                        setHipError(e, "custenCHServerStep (allocation)");
+                        return -1;
+                    }
+                    (void)hipGetLastError();
+                    break;  // wait for memory to free up
+                }
+                if ((e = l->engine->admit(r, r->c0.data(),
+                                          r->cOld0.empty() ? nullptr : r->cOld0.data(),
+                                          l->stream)) != hipSuccess) {
+                    setHipError(e, "custenCHServerStep (admit)");
+                    return -1;
+                }
+                s->used += fp;
+                s->unstaged--;
+            }
+            s->resident++;
+            r->state = 1;
+            r->tAdmit = Clock::now();
+            s->queue.pop_front();
+            admitted++;
         }
-        if ((e = l->engine->admit(r, r->c0.data(), r->cOld0.empty() ? nullptr : r->cOld0.data(),
-                                  l->stream)) != hipSuccess) {
-            setHipError(e, "custenCHServerStep (admit)");
-            return -1;
-        }
-        s->used += fp;
-        s->resident++;
-        r->state = 1;
-        r->tAdmit = Clock::now();
-        s->queue.pop_front();
-    }
-    // enqueue the next batch on every idle lane
-    for (auto& kv : s->lanes) {
-        CHLane& l = *kv.second;
-        if (l.inFlight > 0 || l.engine->active().empty()) continue;
-        int k = kLaneBatch;
-        for (CHRequest* r : l.engine->active()) k = std::min(k, r->steps - r->done);
-        k = std::max(k, 1);
-        for (int i = 0; i < k; i++) {
-            if ((e = l.engine->stepAll(l.stream, false)) != hipSuccess) {
+        // enqueue: a lane takes a batch while it has fewer than two in flight
+        // and every resident request has steps left beyond those in flight;
+        // a lane whose queue head waits for it to drain takes no second batch
+        int enqueued = 0;
+        for (auto& kv : s->lanes) {
+            CHLane& l = *kv.second;
+            if (l.count >= 2 || l.engine->active().empty()) continue;
+            if (l.count > 0 && !s->queue.empty() && s->queue.front()->co.n == l.n) continue;
+            const int flight = l.inFlight();
+            int k = kLaneBatch;
+            for (CHRequest* r : l.engine->active()) k = std::min(k, r->steps - r->done - flight);
+            if (k < 1) continue;
+            if ((e = l.engine->stepBatch(l.stream, k)) != hipSuccess) {
                 setHipError(e, "custenCHServerStep");
                 return -1;
             }
+            const int slot = (l.head + l.count) & 1;
+            if ((e = hipEventRecord(l.done[slot], l.stream)) != hipSuccess) {
+                setHipError(e, "custenCHServerStep");
+                return -1;
+            }
+            l.steps[slot] = k;
+            l.count++;
+            enqueued++;
         }
-        if ((e = hipEventRecord(l.done, l.stream)) != hipSuccess) {
-            setHipError(e, "custenCHServerStep");
-            return -1;
+        const int left = (int)s->queue.size() + s->resident + s->draining;
+        if (retired + admitted + enqueued > 0 || left == 0 || attempt > 0) return left;
+        // nothing to do without waiting: wait for the first batch or result
+        // copy to finish
+        bool anyInFlight = !s->drains.empty();
+        for (auto& kv : s->lanes) anyInFlight = anyInFlight || kv.second->count > 0;
+        if (!anyInFlight) return left;
+        for (;;) {
+            bool ready = false;
+            for (auto& kv : s->lanes) {
+                CHLane& l = *kv.second;
+                if (l.count > 0 && hipEventQuery(l.done[l.head]) != hipErrorNotReady) ready = true;
+            }
+            for (const CHDrain& d : s->drains)
+                if (hipEventQuery(d.done) != hipErrorNotReady) ready = true;
+            if (ready) break;
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
-        l.inFlight = k;
-        l.enqueued = Clock::now();
     }
-    return (int)s->queue.size() + s->resident;
 }
 
 }  // namespace
@@ -1009,7 +1284,8 @@ extern "C" int custenCHServerResult(custenCHServer* s, int id, double* c)
         setError(CUSTEN_ERROR_INVALID_VALUE, "custenCHServerResult: request not finished");
         return CUSTEN_ERROR_INVALID_VALUE;
     }
-    std::memcpy(c, r->result.data(), r->result.size() * sizeof(double));
+    const size_t nn = (size_t)r->co.n * r->co.n;
+    std::memcpy(c, r->pinnedResult ? r->pinnedResult : r->result.data(), nn * sizeof(double));
     return CUSTEN_SUCCESS;
 }
 
@@ -1023,10 +1299,31 @@ extern "C" void custenCHServerDestroy(custenCHServer* s)
         for (CHRequest* r : std::vector<CHRequest*>(l.engine->active())) {
             l.engine->remove(r);
             s->pool->put(r->buf);
+            r->buf = CHBuffers{};
         }
         releaseLane(l);
     }
     s->lanes.clear();
+    for (CHDrain& d : s->drains) {
+        (void)hipEventSynchronize(d.done);
+        (void)hipEventDestroy(d.done);
+        for (CHRequest* r : d.reqs) {
+            s->pool->put(r->buf);
+            r->buf = CHBuffers{};
+        }
+    }
+    s->drains.clear();
+    for (CHRequest* r : s->queue)
+        if (r->staged) {
+            s->pool->put(r->buf);
+            r->buf = CHBuffers{};
+        }
+    for (auto& r : s->all)
+        if (r->pinnedResult) {
+            s->pinned.put(r->pinnedResult, (size_t)r->co.n * r->co.n);
+            r->pinnedResult = nullptr;
+        }
+    if (s->upload) (void)hipStreamDestroy(s->upload);
     s->pool.reset();
     delete s;
 }

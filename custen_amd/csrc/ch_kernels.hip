@@ -206,7 +206,7 @@ __device__ __forceinline__ double divByRecip(double n, double d, double y)
     return __builtin_fma(__builtin_fma(-d, q, n), y, q);
 }
 
-__global__ __launch_bounds__(64) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
+__global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
                                                      int which)
 {
     // ring of diagonals: [slot][array][row of the step]; every lane writes
