@@ -27,6 +27,7 @@ namespace chdev {
 // through them are global (not flat) instructions, which the wait counters
 // track separately from LDS traffic.
 using gdouble = __attribute__((address_space(1))) double;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int findRequest(const CHDesc* d, int count, int item, int which)
 {
@@ -230,57 +231,79 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
     const gdouble* __restrict__ du = dd + m;
     const gdouble* __restrict__ dw = du + m;
     auto at = [&](int row) -> gdouble& { return base[(int64_t)row * s + col]; };
+    // Inside a step every row is addressed as a buffer whose base is the
+    // step's first row (the same for all lanes) plus this lane's byte offset
+    // of row k of the step, so a load or store needs no address arithmetic
+    // of its own.
+    char* const gbase = (char*)(which ? D.X : D.Y);
+    const size_t stride = (size_t)n * sizeof(double);
+    uint32_t off[SR];
+#pragma unroll
+    for (int k = 0; k < SR; k++) off[k] = (uint32_t)col * 8u + (uint32_t)(k * stride);
+    auto rsrc = [](char* p) { return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, 0x7fffffff, 0x00020000); };
+    auto load = [&](__amdgpu_buffer_rsrc_t rs, int k) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off[k], 0, 0));
+    };
+    auto store = [&](__amdgpu_buffer_rsrc_t rs, int k, double v) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, off[k], 0, 0);
+    };
+    const int nFull = (m - 2) / SR;  // whole steps of each sweep
 
     // ---- forward substitution: rows 0 and 1, then rows 2 .. m-1 in steps
     double b2 = at(0) / dd[0];
     at(0) = b2;
     double b1 = (at(1) - dl[1] * b2) / dd[1];
     at(1) = b1;
-    {
-        const int nFull = (m - 2) / SR;  // whole steps
-        auto row = [&](int j, int k) { return min(2 + j * SR + k, m - 1); };
+    if (nFull > 0) {
+        // rows 2 + j SR + k of step j (steps past the sweep: the last one)
+        auto rows = [&](int j) { return rsrc(gbase + (size_t)(2 + min(j, nFull - 1) * SR) * stride); };
         auto loadStep = [&](double (&buf)[SR], int j) {
+            const __amdgpu_buffer_rsrc_t p = rows(j);
 #pragma unroll
-            for (int k = 0; k < SR; k++) buf[k] = at(row(j, k));
+            for (int k = 0; k < SR; k++) buf[k] = load(p, k);
         };
+        // every row of the staged step has a reciprocal (uniform)
+        bool recips = true;
         auto stageDiag = [&](int slot, double p0, double p1, double p2) {
+            const double y = divRecip(p2);
             sd[slot][0][lane] = p0;
             sd[slot][1][lane] = p1;
             sd[slot][2][lane] = p2;
-            sd[slot][3][lane] = divRecip(p2);
+            sd[slot][3][lane] = y;
+            recips = __all(y != 0.0);
         };
+        auto diagRow = [&](int j) { return min(2 + j * SR + min(lane, SR - 1), m - 1); };
         // step j: diagonals of step j+1 and the column rows of step j+2 are
         // loaded, step j is computed from `cur` and ring slot j & 1
         auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
-            const int rd = row(j + 1, lane < SR ? lane : SR - 1);
+            const int rd = diagRow(j + 1);
             const double p0 = ds[rd], p1 = dl[rd], p2 = dd[rd];
             loadStep(ahead, j + 2);
             const int slot = j & 1;
-            double nmin = 0x1p900, nmax = 0x1p-900, ymin = 1.0;
+            const __amdgpu_buffer_rsrc_t out = rows(j);
+            double nmin = 0x1p900, nmax = 0x1p-900;
             const double b2s = b2, b1s = b1;
 #pragma unroll
             for (int k = 0; k < SR; k++) {
-                const double y = sd[slot][3][k];
                 const double t = cur[k] - sd[slot][0][k] * b2;
                 const double num = t - sd[slot][1][k] * b1;
-                const double v = divByRecip(num, sd[slot][2][k], y);
+                const double v = divByRecip(num, sd[slot][2][k], sd[slot][3][k]);
                 nmin = __builtin_fmin(nmin, __builtin_fabs(num));
                 nmax = __builtin_fmax(nmax, __builtin_fabs(num));
-                ymin = __builtin_fmin(ymin, __builtin_fabs(y));
-                at(2 + j * SR + k) = v;
+                store(out, k, v);
                 b2 = b1;
                 b1 = v;
             }
             // a numerator outside divByRecip's range (zero, whose sign the
             // corrections would lose, subnormal, huge, Inf) or a row without
             // a reciprocal: this lane repeats the step with IEEE division
-            if (__builtin_expect(!(nmin >= 0x1p-900 && nmax <= 0x1p900 && ymin != 0.0), 0)) {
+            if (__builtin_expect(!(nmin >= 0x1p-900 && nmax <= 0x1p900 && recips), 0)) {
                 b2 = b2s;
                 b1 = b1s;
                 for (int k = 0; k < SR; k++) {
                     const double t = cur[k] - sd[slot][0][k] * b2;
                     const double v = (t - sd[slot][1][k] * b1) / sd[slot][2][k];
-                    at(2 + j * SR + k) = v;
+                    store(out, k, v);
                     b2 = b1;
                     b1 = v;
                 }
@@ -289,7 +312,7 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         };
         double bufA[SR], bufB[SR], bufC[SR];
         {
-            const int r0 = row(0, lane < SR ? lane : SR - 1);
+            const int r0 = diagRow(0);
             stageDiag(0, ds[r0], dl[r0], dd[r0]);
         }
         loadStep(bufA, 0);
@@ -302,14 +325,14 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         }
         if (j < nFull) step(j++, bufA, bufC);
         if (j < nFull) step(j++, bufB, bufA);
-        // rows left after the whole steps, one at a time
-        for (int i = 2 + nFull * SR; i < m; i++) {
-            const double t = at(i) - ds[i] * b2;
-            const double v = (t - dl[i] * b1) / dd[i];
-            at(i) = v;
-            b2 = b1;
-            b1 = v;
-        }
+    }
+    // rows left after the whole steps, one at a time
+    for (int i = 2 + nFull * SR; i < m; i++) {
+        const double t = at(i) - ds[i] * b2;
+        const double v = (t - dl[i] * b1) / dd[i];
+        at(i) = v;
+        b2 = b1;
+        b1 = v;
     }
 
     // ---- backward substitution: b1 = x[m-1], b2 = b[m-2]; then rows m-3
@@ -319,27 +342,33 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
     double a0 = b2 - du[m - 2] * a1;
     at(m - 2) = a0;
     const double xm2 = a0;
-    {
-        const int nFull = (m - 2) / SR;
-        auto row = [&](int j, int k) { return max(m - 3 - j * SR - k, 0); };
+    if (nFull > 0) {
+        // step j holds rows m-3 - j SR - k; row k of the step sits at offset
+        // SR-1-k above the step's lowest row m-3 - j SR - (SR-1)
+        auto rows = [&](int j) {
+            return rsrc(gbase + (size_t)(m - 3 - min(j, nFull - 1) * SR - (SR - 1)) * stride);
+        };
         auto loadStep = [&](double (&buf)[SR], int j) {
+            const __amdgpu_buffer_rsrc_t p = rows(j);
 #pragma unroll
-            for (int k = 0; k < SR; k++) buf[k] = at(row(j, k));
+            for (int k = 0; k < SR; k++) buf[k] = load(p, SR - 1 - k);
         };
         auto stageDiag = [&](int slot, double p0, double p1) {
             sd[slot][0][lane] = p0;
             sd[slot][1][lane] = p1;
         };
+        auto diagRow = [&](int j) { return max(m - 3 - j * SR - min(lane, SR - 1), 0); };
         auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
-            const int rd = row(j + 1, lane < SR ? lane : SR - 1);
+            const int rd = diagRow(j + 1);
             const double p0 = du[rd], p1 = dw[rd];
             loadStep(ahead, j + 2);
             const int slot = j & 1;
+            const __amdgpu_buffer_rsrc_t out = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) {
                 const double t = cur[k] - sd[slot][0][k] * a0;
                 const double v = t - sd[slot][1][k] * a1;
-                at(m - 3 - j * SR - k) = v;
+                store(out, SR - 1 - k, v);
                 a1 = a0;
                 a0 = v;
             }
@@ -347,7 +376,7 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         };
         double bufA[SR], bufB[SR], bufC[SR];
         {
-            const int r0 = row(0, lane < SR ? lane : SR - 1);
+            const int r0 = diagRow(0);
             stageDiag(0, du[r0], dw[r0]);
         }
         loadStep(bufA, 0);
@@ -360,13 +389,13 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         }
         if (j < nFull) step(j++, bufA, bufC);
         if (j < nFull) step(j++, bufB, bufA);
-        for (int i = m - 3 - nFull * SR; i >= 0; i--) {
-            const double t = at(i) - du[i] * a0;
-            const double v = t - dw[i] * a1;
-            at(i) = v;
-            a1 = a0;
-            a0 = v;
-        }
+    }
+    for (int i = m - 3 - nFull * SR; i >= 0; i--) {
+        const double t = at(i) - du[i] * a0;
+        const double v = t - dw[i] * a1;
+        at(i) = v;
+        a1 = a0;
+        a0 = v;
     }
     if (!live) return;
     // solveEnd: a0 = x[0], a1 = x[1]
