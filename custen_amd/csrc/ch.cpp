@@ -1181,10 +1181,11 @@ int serverIteration(custenCHServer* s)
         }
         // enqueue: a lane takes a batch while it has fewer than two in flight
         // and every resident request has steps left beyond those in flight;
-        // a lane whose queue head waits for it to drain takes no second batch
+        // a lane whose queue head waits for it to drain takes no second batch.
+        // Largest grids first: their lane sets the pace of a mixed load.
         int enqueued = 0;
-        for (auto& kv : s->lanes) {
-            CHLane& l = *kv.second;
+        for (auto it = s->lanes.rbegin(); it != s->lanes.rend(); ++it) {
+            CHLane& l = *it->second;
             if (l.count >= 2 || l.engine->active().empty()) continue;
             if (l.count > 0 && !s->queue.empty() && s->queue.front()->co.n == l.n) continue;
             const int flight = l.inFlight();
@@ -1273,6 +1274,32 @@ extern "C" int custenCHServerQuery(custenCHServer* s, int id, int* state, double
     return CUSTEN_SUCCESS;
 }
 
+namespace {
+
+// Host copy of a result into the caller's array.  A large copy into memory
+// the caller has not touched yet is bound by page faults, which the kernel
+// serves in parallel: it is split over up to 8 threads.
+void copyOut(double* dst, const double* src, size_t count)
+{
+    constexpr size_t kSplitBytes = size_t(8) << 20;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned parts = std::max(1u, std::min(8u, hw));
+    if (count * sizeof(double) < kSplitBytes || parts == 1) {
+        std::memcpy(dst, src, count * sizeof(double));
+        return;
+    }
+    const size_t chunk = (count + parts - 1) / parts;
+    std::vector<std::thread> workers;
+    for (unsigned t = 1; t < parts && t * chunk < count; t++) {
+        const size_t b = t * chunk, e = std::min(count, b + chunk);
+        workers.emplace_back([=]() { std::memcpy(dst + b, src + b, (e - b) * sizeof(double)); });
+    }
+    std::memcpy(dst, src, std::min(chunk, count) * sizeof(double));
+    for (std::thread& w : workers) w.join();
+}
+
+}  // namespace
+
 extern "C" int custenCHServerResult(custenCHServer* s, int id, double* c)
 {
     if (s == nullptr || id < 0 || id >= (int)s->all.size() || c == nullptr) {
@@ -1285,7 +1312,7 @@ extern "C" int custenCHServerResult(custenCHServer* s, int id, double* c)
         return CUSTEN_ERROR_INVALID_VALUE;
     }
     const size_t nn = (size_t)r->co.n * r->co.n;
-    std::memcpy(c, r->pinnedResult ? r->pinnedResult : r->result.data(), nn * sizeof(double));
+    copyOut(c, r->pinnedResult ? r->pinnedResult : r->result.data(), nn);
     return CUSTEN_SUCCESS;
 }
 
