@@ -255,11 +255,9 @@ public:
         auto sync = [&]() { return plain ? hipStreamSynchronize(st) : hipSuccess; };
         if ((e = chLaunchRhs(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 0, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
         if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 1, st)) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
         if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
         if ((e = sync()) != hipSuccess) return e;
@@ -330,9 +328,8 @@ private:
     {
         hipError_t e;
         if ((e = chLaunchRhs(dDescs_, count_, tiles_, step, st)) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 0, st)) != hipSuccess) return e;
         if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, 1, st)) != hipSuccess) return e;
+        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, st)) != hipSuccess) return e;
         if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
         return chLaunchFinal(dDescs_, count_, tiles_, step, st);
     }

@@ -22,8 +22,8 @@ struct CHDesc {
     double* f[2];       // the two time levels; at engine step s the current field
                         // is f[(s - admitStep) & 1] and the previous one the other;
                         // the final phase writes the new field over the previous one
-    double* X;          // right-hand side / solution work, n*n
-    double* Y;          // transposed work buffer, n*n (x-direction solves)
+    double* X;          // solution work, n*n (y-direction solves)
+    double* Y;          // transposed right-hand side, n*n (x-direction solves)
     const double* fac;  // factored diagonals + inverses: ds, dl, d, du, dw, inv1, inv2 (7 * m)
     double a, b, d, e;  // matrix diagonals (solveEnd)
     double omega[4];
@@ -50,15 +50,13 @@ CUSTEN_HD inline int chTrTiles(int n) { const int t = (n + kCHTr - 1) / kCHTr; r
 // The phases of one step for `count` requests (descs in device memory; the
 // bases are filled so that each phase's work items are contiguous per
 // request):
-//   rhs        X = RHS(c, cOld)
-//   transpose  Y = X^T                                   (corr = 0)
+//   rhs        Y = RHS(c, cOld)^T (the right-hand side, written transposed)
 //   colSolve   cyclic solve of every column of Y          (which = 0)
-//   transpose  X = (Y with the column solveFull applied)^T (corr = 1)
+//   transpose  X = (Y with the column solveFull applied)^T
 //   colSolve   cyclic solve of every column of X          (which = 1)
 //   final      c_new = cBar + (X with the column solveFull applied)
 hipError_t chLaunchRhs(const CHDesc* descs, int count, int totalTiles, int step, hipStream_t st);
-hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, int corr,
-                             hipStream_t st);
+hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, hipStream_t st);
 hipError_t chLaunchColSolve(const CHDesc* descs, int count, int totalColBlocks, int which,
                             hipStream_t st);
 hipError_t chLaunchFinal(const CHDesc* descs, int count, int totalTiles, int step, hipStream_t st);

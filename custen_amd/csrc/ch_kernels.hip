@@ -4,14 +4,16 @@
 // One step of cuPentCahnADI (cuPentCahnADI/src/cuPentCahnADI.cu:528-596;
 // serial form cuPentSpeedUp/serialCahnADITiming/serialCahnADI.c:1010-1047)
 // in four phases:
-//   rhs   : cBar = 2c - cOld, X = L5x5(cBar) + ((-2/3)(c - cOld) + L3x3(c^3 - c))
-//   xsolve: Y = X^T, cyclic pentadiagonal solve of every column of Y
-//           (pentSolve + solveEnd), X = Y^T with solveFull applied on the way
+//   rhs   : cBar = 2c - cOld, Y = (L5x5(cBar) + ((-2/3)(c - cOld) + L3x3(c^3 - c)))^T
+//           (the right-hand side is written transposed, through LDS)
+//   xsolve: cyclic pentadiagonal solve of every column of Y (pentSolve +
+//           solveEnd), X = Y^T with solveFull applied on the way
 //   ysolve: cyclic solve of every column of X (pentSolve + solveEnd)
 //   final : c_new = cBar + (X - (inv1 x~2 + inv2 x~1)), written over cOld
 // Every solve is a thread-per-column sweep (coalesced across the wave) with a
-// block of row loads in flight ahead of the recurrence; the transposes
-// replace the reference's cublasDgeam calls (cuPentCahnADI.cu:552, 566).
+// block of row loads in flight ahead of the recurrence; the transposed store
+// of the right-hand side and the transpose after the x solve replace the
+// reference's cublasDgeam calls (cuPentCahnADI.cu:552, 566).
 // Compiled with -ffp-contract=off: every operation is rounded as written, the
 // stencil sums are explicit fma chains (the cuSten order).
 #include <hip/hip_runtime.h>
@@ -61,6 +63,7 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
 {
     __shared__ double cb[CBH * CBW];
     __shared__ double q[QH * QW];
+    __shared__ double ot[RX][RY + 1];  // the tile's result, [x][y]
     const int item = blockIdx.x;
     const int r = findRequest(descs, count, item, 0);
     const CHDesc& D = descs[r];
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
     const int par = (step - D.admitStep) & 1;
     const gdouble* __restrict__ c = (const gdouble*)D.f[par];
     const gdouble* __restrict__ co = (const gdouble*)D.f[par ^ 1];
-    gdouble* __restrict__ Xo = (gdouble*)D.X;
+    gdouble* __restrict__ Yo = (gdouble*)D.Y;
     const int local = item - D.tileBase;
     const int tilesX = (n + RX - 1) / RX;
     const int x0 = (local % tilesX) * RX;
@@ -114,17 +117,25 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
         const double t1 = c[k] - co[k];
         const double t2 = k23 * t1;
         const double t3 = t2 + nl;
-        Xo[k] = lin + t3;
+        ot[lane][ly] = lin + t3;
+    }
+    __syncthreads();
+    // the tile goes out transposed (Y = X^T), 16 consecutive y per x: each
+    // store instruction of a wave writes four whole 128-byte segments
+#pragma unroll
+    for (int i = 0; i < RX * RY / 256; i++) {
+        const int idx = i * 256 + (int)threadIdx.x;
+        const int xl = idx / RY, yl = idx % RY;
+        if (x0 + xl < n && y0 + yl < n) Yo[(int64_t)(x0 + xl) * n + y0 + yl] = ot[xl][yl];
     }
 }
 
 // ---------------------------------------------------------------------------
-// transpose (64 x 64 tiles through LDS).  corr = 0: Y = X^T.  corr = 1:
-// X = Y'^T where Y' is Y with the column solveFull of the x-direction solves
-// applied (serialCahnADI.c:661-678): y_j -= inv1[j] y~2 + inv2[j] y~1, j < n-2.
+// transpose (64 x 64 tiles through LDS): X = Y'^T where Y' is Y with the
+// column solveFull of the x-direction solves applied (serialCahnADI.c:661-678):
+// y_j -= inv1[j] y~2 + inv2[j] y~1, j < n-2.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict__ descs, int count,
-                                                       int corr)
+__global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict__ descs, int count)
 {
     __shared__ double t[kCHTr][kCHTr + 1];
     const int item = blockIdx.x;
@@ -135,14 +146,14 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
     const int local = item - D.trBase;
     const int R0 = (local / tiles) * kCHTr;  // source rows
     const int C0 = (local % tiles) * kCHTr;  // source columns
-    const gdouble* __restrict__ src = (const gdouble*)(corr ? D.Y : D.X);
-    gdouble* __restrict__ dst = (gdouble*)(corr ? D.X : D.Y);
+    const gdouble* __restrict__ src = (const gdouble*)D.Y;
+    gdouble* __restrict__ dst = (gdouble*)D.X;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = C0 + lane;
     double e2 = 0.0, e1 = 0.0;
     const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
     const gdouble* inv2 = inv1 + m;
-    if (corr && c < n) {
+    if (c < n) {
         e2 = src[(int64_t)(n - 2) * n + c];
         e1 = src[(int64_t)(n - 1) * n + c];
     }
@@ -151,7 +162,7 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
         const int row = R0 + k;
         if (row < n && c < n) {
             double v = src[(int64_t)row * n + c];
-            if (corr && row < m) v = v - (inv1[row] * e2 + inv2[row] * e1);
+            if (row < m) v = v - (inv1[row] * e2 + inv2[row] * e1);
             t[k][lane] = v;
         }
     }
@@ -453,12 +464,10 @@ hipError_t chLaunchRhs(const CHDesc* descs, int count, int totalTiles, int step,
     return hipGetLastError();
 }
 
-hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, int corr,
-                             hipStream_t st)
+hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, hipStream_t st)
 {
     if (count <= 0 || totalTiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chdev::transposeKernel, dim3(totalTiles), dim3(256), 0, st, descs, count,
-                       corr);
+    hipLaunchKernelGGL(chdev::transposeKernel, dim3(totalTiles), dim3(256), 0, st, descs, count);
     return hipGetLastError();
 }
 

@@ -163,9 +163,8 @@ extern "C" int custenCHAdvanceNoAlloc(int deviceNum, const custenCHParams* param
     const int tiles = chTiles(n), trTiles = chTrTiles(n), colBlocks = chColBlocks(n);
     for (int s = 0; s < steps; s++) {
         if ((e = chLaunchRhs(dDesc, 1, tiles, s, stream)) != hipSuccess ||
-            (e = chLaunchTranspose(dDesc, 1, trTiles, 0, stream)) != hipSuccess ||
             (e = chLaunchColSolve(dDesc, 1, colBlocks, 0, stream)) != hipSuccess ||
-            (e = chLaunchTranspose(dDesc, 1, trTiles, 1, stream)) != hipSuccess ||
+            (e = chLaunchTranspose(dDesc, 1, trTiles, stream)) != hipSuccess ||
             (e = chLaunchColSolve(dDesc, 1, colBlocks, 1, stream)) != hipSuccess ||
             (e = chLaunchFinal(dDesc, 1, tiles, s, stream)) != hipSuccess) {
             setHipError(e, fn);
