@@ -79,9 +79,12 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
 
     for (int idx = threadIdx.x; idx < CBH * CBW; idx += 256) {
         const int rr = idx / CBW, cc = idx - rr * CBW;
-        int gy = y0 - 2 + rr, gx = x0 - 2 + cc;
-        gy = ((gy % n) + n) % n;
-        gx = ((gx % n) + n) % n;
+        // the points an output reads lie in [-2, n + 1], where one conditional
+        // add or subtract wraps (n >= 8); staged points past n + 1 (a tile
+        // wider or taller than what is left of the grid) are never read and
+        // are clamped to n + 1
+        const int gy = wrapN(min(y0 - 2 + rr, n + 1), n);
+        const int gx = wrapN(min(x0 - 2 + cc, n + 1), n);
         const int64_t k = (int64_t)gy * n + gx;
         const double cv = c[k];
         const double ov = co[k];
