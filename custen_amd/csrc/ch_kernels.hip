@@ -77,7 +77,14 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
     const int x0 = (local % tilesX) * RX;
     const int y0 = (local / tilesX) * RY;
 
-    for (int idx = threadIdx.x; idx < CBH * CBW; idx += 256) {
+    // Staging: every thread issues all of its loads before it uses any (a
+    // thread's share past the tile repeats the tile's last point, so the
+    // loads are unconditional and all in flight together).
+    constexpr int kStage = (CBH * CBW + 255) / 256;
+    double cv[kStage], ov[kStage];
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int idx = min((int)threadIdx.x + i * 256, CBH * CBW - 1);
         const int rr = idx / CBW, cc = idx - rr * CBW;
         // the points an output reads lie in [-2, n + 1], where one conditional
         // add or subtract wraps (n >= 8); staged points past n + 1 (a tile
@@ -86,20 +93,34 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
         const int gy = wrapN(min(y0 - 2 + rr, n + 1), n);
         const int gx = wrapN(min(x0 - 2 + cc, n + 1), n);
         const int64_t k = (int64_t)gy * n + gx;
-        const double cv = c[k];
-        const double ov = co[k];
-        const double t = 2.0 * cv;
-        cb[idx] = t - ov;
+        cv[i] = c[k];
+        ov[i] = co[k];
+    }
+    // this thread's own points (4 rows of column x0 + lane), for c - cOld
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = x0 + lane;
+    double cOwn[4], oOwn[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int64_t k = (int64_t)min(y0 + wave * 4 + rr, n - 1) * n + min(x, n - 1);
+        cOwn[rr] = c[k];
+        oOwn[rr] = co[k];
+    }
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int idx = (int)threadIdx.x + i * 256;
+        if (idx >= CBH * CBW) break;
+        const int rr = idx / CBW, cc = idx - rr * CBW;
+        const double t = 2.0 * cv[i];
+        cb[idx] = t - ov[i];
         if (rr >= 1 && rr < CBH - 1 && cc >= 1 && cc < CBW - 1) {
-            const double u2 = cv * cv;
-            const double u3 = u2 * cv;
-            q[(rr - 1) * QW + (cc - 1)] = u3 - cv;
+            const double u2 = cv[i] * cv[i];
+            const double u3 = u2 * cv[i];
+            q[(rr - 1) * QW + (cc - 1)] = u3 - cv[i];
         }
     }
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = x0 + lane;
     const double k23 = -(2.0 / 3.0);
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
@@ -116,8 +137,7 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
         for (int j = 0; j < 3; j++)
 #pragma unroll
             for (int i = 0; i < 3; i++) nl = __builtin_fma(D.wnl[j * 3 + i], q[(ly + j) * QW + lane + i], nl);
-        const int64_t k = (int64_t)y * n + x;
-        const double t1 = c[k] - co[k];
+        const double t1 = cOwn[rr] - oOwn[rr];
         const double t2 = k23 * t1;
         const double t3 = t2 + nl;
         ot[lane][ly] = lin + t3;
@@ -153,20 +173,26 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
     gdouble* __restrict__ dst = (gdouble*)D.X;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = C0 + lane;
-    double e2 = 0.0, e1 = 0.0;
     const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
     const gdouble* inv2 = inv1 + m;
-    if (c < n) {
-        e2 = src[(int64_t)(n - 2) * n + c];
-        e1 = src[(int64_t)(n - 1) * n + c];
-    }
-#pragma unroll 4
-    for (int k = wave; k < kCHTr; k += 4) {
+    // all loads first (points past the grid repeat its last row or column),
+    // then the corrections and the guarded LDS writes
+    constexpr int kRowsPerWave = kCHTr / 4;
+    const int cc = min(c, n - 1);
+    const double e2 = src[(int64_t)(n - 2) * n + cc];
+    const double e1 = src[(int64_t)(n - 1) * n + cc];
+    double v[kRowsPerWave];
+#pragma unroll
+    for (int i = 0; i < kRowsPerWave; i++)
+        v[i] = src[(int64_t)min(R0 + wave + 4 * i, n - 1) * n + cc];
+#pragma unroll
+    for (int i = 0; i < kRowsPerWave; i++) {
+        const int k = wave + 4 * i;
         const int row = R0 + k;
         if (row < n && c < n) {
-            double v = src[(int64_t)row * n + c];
-            if (row < m) v = v - (inv1[row] * e2 + inv2[row] * e1);
-            t[k][lane] = v;
+            double w = v[i];
+            if (row < m) w = w - (inv1[row] * e2 + inv2[row] * e1);
+            t[k][lane] = w;
         }
     }
     __syncthreads();
@@ -445,16 +471,29 @@ __global__ __launch_bounds__(256) void finalKernel(const CHDesc* __restrict__ de
     if (x >= n) return;
     const double e2 = X[(int64_t)(n - 2) * n + x];
     const double e1 = X[(int64_t)(n - 1) * n + x];
+    // all loads first (rows past the grid repeat its last row), then the
+    // guarded stores
+    double xv[4], cv[4], ov[4], i1[4], i2[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int y = min(y0 + wave * 4 + rr, n - 1);
+        const int64_t k = (int64_t)y * n + x;
+        xv[rr] = X[k];
+        cv[rr] = c[k];
+        ov[rr] = co[k];
+        const int yi = min(y, m - 1);
+        i1[rr] = inv1[yi];
+        i2[rr] = inv2[yi];
+    }
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
         const int y = y0 + wave * 4 + rr;
         if (y >= n) break;
-        const int64_t k = (int64_t)y * n + x;
-        double xf = X[k];
-        if (y < m) xf = xf - (inv1[y] * e2 + inv2[y] * e1);
-        const double t = 2.0 * c[k];
-        const double cBar = t - co[k];
-        co[k] = cBar + xf;
+        double xf = xv[rr];
+        if (y < m) xf = xf - (i1[rr] * e2 + i2[rr] * e1);
+        const double t = 2.0 * cv[rr];
+        const double cBar = t - ov[rr];
+        co[(int64_t)y * n + x] = cBar + xf;
     }
 }
 
