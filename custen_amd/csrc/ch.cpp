@@ -273,7 +273,7 @@ public:
     // kernels use a step number only through its parity relative to each
     // request's admission, so a graph serves every batch whose first step has
     // the parity it was captured at.  Replaying one graph costs the host one
-    // launch instead of 6 k.  Consecutive batches alternate between two
+    // launch instead of 5 k.  Consecutive batches alternate between two
     // graphs: launching a graph whose previous launch is still running makes
     // the host wait for it.
     hipError_t stepBatch(hipStream_t st, int k)

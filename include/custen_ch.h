@@ -31,9 +31,9 @@ typedef struct {
 /* Single-simulation solver with resident fields.  c0: n*n initial condition
  * (host or device); cOld starts equal to c0 as in the reference. */
 custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0);
-/* Advance `steps` steps.  mode 0: the four step kernels are queued back to
- * back; mode 1: the same kernels with a stream synchronisation after each
- * launch (the reference's launch-and-synchronise loop, cuPentCahnADI.cu:528-596),
+/* Advance `steps` steps.  mode 0: the five fused step kernels are queued back
+ * to back; mode 1: the reference's kernel sequence with a stream
+ * synchronisation after each launch (the reference's launch-and-synchronise loop, cuPentCahnADI.cu:528-596),
  * used as the plain per-grid baseline. */
 int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream);
 /* Copy the current field (n*n doubles) to host memory. */
@@ -57,13 +57,19 @@ int custenCHSnapshot(custenCH* ch, const char* path);
 int custenCHCheckpoint(custenCH* ch, const char* path);
 custenCH* custenCHRestore(int deviceNum, const char* path);
 
-/* Request server with continuous batching.  Requests are admitted while their
- * resident footprint fits in `hbmBudgetBytes` and a slot is free; every engine
- * iteration advances all resident requests by one step with one batched
- * launch sequence; finished requests retire and free their slot. */
+/* Request server with continuous batching.  Requests are admitted (FIFO)
+ * while their resident footprint fits in `hbmBudgetBytes` and a slot is free.
+ * Requests of one grid size share a lane (one stream) whose batched launches
+ * advance all of them together, up to 8 steps per batch and two batches in
+ * flight; lanes of different sizes run concurrently.  Finished requests
+ * retire and free their slot; their results are copied to host memory
+ * asynchronously. */
 typedef struct custenCHServer custenCHServer;
 custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetBytes, int maxSlots);
-/* Queue a request.  Returns its id (>= 0) or -1.  c0 is copied at submit. */
+/* Queue a request.  Returns its id (>= 0) or -1.  c0 is uploaded to the
+ * request's device buffers at submit when the budget has room (otherwise
+ * copied to host memory until admission); the caller may reuse c0 as soon as
+ * Submit returns. */
 int custenCHServerSubmit(custenCHServer* s, const custenCHParams* params, int steps,
                          const double* c0);
 /* The same for a simulation resumed from a checkpoint: both time levels. */
