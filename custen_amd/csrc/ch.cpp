@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -863,6 +864,7 @@ struct custenCHServer {
     int unstaged = 0;     // queued requests whose input waits in host memory
     int draining = 0;     // requests whose result is being copied out
     size_t pinnedBytes = 0;  // results held in pinned memory
+    size_t pinnedCap = 0;    // ... up to this many bytes
     hipStream_t upload = nullptr;
     std::map<int, std::unique_ptr<CHLane>> lanes;
     std::unique_ptr<BufferPool> pool;
@@ -876,8 +878,10 @@ namespace {
 
 // Steps a lane enqueues at once (fewer if a request finishes earlier).
 constexpr int kLaneBatch = 8;
-// Finished results kept in pinned memory up to this many bytes; later ones
-// move to pageable memory once their copy has ended.
+// Finished results kept in pinned memory up to this many bytes (the
+// environment variable CUSTEN_CH_PINNED_RESULT_BYTES, read when a server is
+// created, overrides it); later ones move to pageable memory once their copy
+// has ended.
 constexpr size_t kPinnedResultBytes = size_t(2) << 30;
 
 void releaseLane(CHLane& l)
@@ -933,6 +937,9 @@ extern "C" custenCHServer* custenCHServerCreate(int deviceNum, size_t hbmBudgetB
     s->budget = hbmBudgetBytes;
     s->maxSlots = maxSlots;
     s->pool.reset(new BufferPool(hbmBudgetBytes / 4));
+    s->pinnedCap = kPinnedResultBytes;
+    if (const char* v = std::getenv("CUSTEN_CH_PINNED_RESULT_BYTES"))
+        s->pinnedCap = (size_t)std::strtoull(v, nullptr, 10);
     if ((e = hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking)) != hipSuccess) {
         setHipError(e, "custenCHServerCreate");
         return nullptr;
@@ -1088,7 +1095,7 @@ int finishDrains(custenCHServer* s)
             finishRequest(s, r);
             s->draining--;
             finished++;
-            if (s->pinnedBytes > kPinnedResultBytes) {
+            if (s->pinnedBytes > s->pinnedCap) {
                 const size_t nn = (size_t)r->co.n * r->co.n;
                 r->result.assign(r->pinnedResult, r->pinnedResult + nn);
                 s->pinned.put(r->pinnedResult, nn);

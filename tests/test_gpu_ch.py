@@ -74,6 +74,27 @@ def test_server_alone_vs_mix(gpu):
         assert mismatches(mix[r], rc.reshape(n, n)) == 0
 
 
+def test_server_results_moved_to_pageable(gpu, monkeypatch):
+    """With no room for results in pinned memory every finished result moves
+    to pageable memory after its copy; the results are unchanged and can be
+    read more than once."""
+    monkeypatch.setenv("CUSTEN_CH_PINNED_RESULT_BYTES", "0")
+    sizes = [32, 48, 32, 64]
+    steps = [3, 5, 4, 2]
+    ics = [ic(n, 70 + r) for r, n in enumerate(sizes)]
+    srv = api.CHServer(budget_bytes=4 * api.ch_footprint(64), max_slots=4)
+    ids = [srv.submit(api.ch_params(n), k, c) for n, k, c in zip(sizes, steps, ics)]
+    srv.run_until_idle()
+    for r, (n, k, c) in enumerate(zip(sizes, steps, ics)):
+        first = srv.result(ids[r])
+        assert bits_equal(first, srv.result(ids[r]))
+        ref = oracle.CH(n)
+        rc, rco = c.ravel().copy(), c.ravel().copy()
+        ref.step(rc, rco, steps=k)
+        assert mismatches(first, rc.reshape(n, n)) == 0
+    srv.close()
+
+
 def test_server_alone_vs_mix_config_sizes(gpu):
     """A mix that includes the served workload's 1024 and 2048 grids: each
     result is the same alone and in the mix, and equals the oracle."""
