@@ -181,17 +181,21 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
     const int cc = min(c, n - 1);
     const double e2 = src[(int64_t)(n - 2) * n + cc];
     const double e1 = src[(int64_t)(n - 1) * n + cc];
-    double v[kRowsPerWave];
+    double v[kRowsPerWave], i1[kRowsPerWave], i2[kRowsPerWave];
 #pragma unroll
-    for (int i = 0; i < kRowsPerWave; i++)
-        v[i] = src[(int64_t)min(R0 + wave + 4 * i, n - 1) * n + cc];
+    for (int i = 0; i < kRowsPerWave; i++) {
+        const int row = R0 + wave + 4 * i;
+        v[i] = src[(int64_t)min(row, n - 1) * n + cc];
+        i1[i] = inv1[min(row, m - 1)];
+        i2[i] = inv2[min(row, m - 1)];
+    }
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) {
         const int k = wave + 4 * i;
         const int row = R0 + k;
         if (row < n && c < n) {
             double w = v[i];
-            if (row < m) w = w - (inv1[row] * e2 + inv2[row] * e1);
+            if (row < m) w = w - (i1[i] * e2 + i2[i] * e1);
             t[k][lane] = w;
         }
     }
