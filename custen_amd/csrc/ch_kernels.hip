@@ -122,25 +122,36 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
     __syncthreads();
 
     const double k23 = -(2.0 / 3.0);
+    // The four rows of this lane's column are summed together, weight by
+    // weight (each row's chain keeps the cuSten order: j outer, i inner), so
+    // each weight is fetched once for the four rows.  Points past the grid
+    // are computed from in-bounds LDS values and never stored.
+    const int ly0 = wave * 4;
+    double lin[4] = {0.0, 0.0, 0.0, 0.0}, nl[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const double w = D.wlin[j * 5 + i];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                lin[rr] = __builtin_fma(w, cb[(ly0 + rr + j) * CBW + lane + i], lin[rr]);
+        }
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double w = D.wnl[j * 3 + i];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                nl[rr] = __builtin_fma(w, q[(ly0 + rr + j) * QW + lane + i], nl[rr]);
+        }
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-        const int ly = wave * 4 + rr;
-        const int y = y0 + ly;
-        if (x >= n || y >= n) continue;
-        double lin = 0.0;
-#pragma unroll
-        for (int j = 0; j < 5; j++)
-#pragma unroll
-            for (int i = 0; i < 5; i++) lin = __builtin_fma(D.wlin[j * 5 + i], cb[(ly + j) * CBW + lane + i], lin);
-        double nl = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-            for (int i = 0; i < 3; i++) nl = __builtin_fma(D.wnl[j * 3 + i], q[(ly + j) * QW + lane + i], nl);
         const double t1 = cOwn[rr] - oOwn[rr];
         const double t2 = k23 * t1;
-        const double t3 = t2 + nl;
-        ot[lane][ly] = lin + t3;
+        const double t3 = t2 + nl[rr];
+        ot[lane][ly0 + rr] = lin[rr] + t3;
     }
     __syncthreads();
     // the tile goes out transposed (Y = X^T), 16 consecutive y per x: each
