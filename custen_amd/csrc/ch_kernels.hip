@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 // compiler computes are exact and no wait drains the loads still in flight.
 // Steps that do not fit whole are done one row at a time after the loop.
 // ---------------------------------------------------------------------------
-constexpr int SR = 16;  // rows per step
+constexpr int SR = 16;  // rows per step (settleStep takes 16)
 
 // Reciprocal of a diagonal entry for the division below: RN(1/d) when
 // 2^-60 <= |d| <= 2^60, else 0 (the row then divides with IEEE division).
@@ -260,6 +260,17 @@ __device__ __forceinline__ double divByRecip(double n, double d, double y)
     double q = n * y;
     q = __builtin_fma(__builtin_fma(-d, q, n), y, q);
     return __builtin_fma(__builtin_fma(-d, q, n), y, q);
+}
+
+// One wait for a whole step of rows: the empty statement takes all SR
+// values as operands, so the compiler waits once for the loads that fill
+// them (issued two steps earlier, so the wait is normally already satisfied)
+// instead of once before each row's first use.
+__device__ __forceinline__ void settleStep(const double (&b)[16])
+{
+    asm volatile("" ::"v"(b[0]), "v"(b[1]), "v"(b[2]), "v"(b[3]), "v"(b[4]), "v"(b[5]), "v"(b[6]),
+                 "v"(b[7]), "v"(b[8]), "v"(b[9]), "v"(b[10]), "v"(b[11]), "v"(b[12]), "v"(b[13]),
+                 "v"(b[14]), "v"(b[15]));
 }
 
 __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict__ descs, int count,
@@ -311,7 +322,11 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
     at(1) = b1;
     if (nFull > 0) {
         // rows 2 + j SR + k of step j (steps past the sweep: the last one)
-        auto rows = [&](int j) { return rsrc(gbase + (size_t)(2 + min(j, nFull - 1) * SR) * stride); };
+        const size_t stepBytes = SR * stride, lastRow = (size_t)(2 + (nFull - 1) * SR) * stride;
+        auto rows = [&](int j) {
+            const size_t o = 2 * stride + (size_t)j * stepBytes;
+            return rsrc(gbase + (o < lastRow ? o : lastRow));
+        };
         auto loadStep = [&](double (&buf)[SR], int j) {
             const __amdgpu_buffer_rsrc_t p = rows(j);
 #pragma unroll
@@ -334,6 +349,7 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             const int rd = diagRow(j + 1);
             const double p0 = ds[rd], p1 = dl[rd], p2 = dd[rd];
             loadStep(ahead, j + 2);
+            settleStep(cur);
             const int slot = j & 1;
             const __amdgpu_buffer_rsrc_t out = rows(j);
             double nmin = 0x1p900, nmax = 0x1p-900;
@@ -400,8 +416,14 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
     if (nFull > 0) {
         // step j holds rows m-3 - j SR - k; row k of the step sits at offset
         // SR-1-k above the step's lowest row m-3 - j SR - (SR-1)
+        // (offsets counted down from the first step's lowest row, clamped at
+        // the last step's)
+        const size_t stepBytes = SR * stride;
+        const size_t firstRow = (size_t)(m - 3 - (SR - 1)) * stride;
+        const size_t lastBack = (size_t)(nFull - 1) * stepBytes;
         auto rows = [&](int j) {
-            return rsrc(gbase + (size_t)(m - 3 - min(j, nFull - 1) * SR - (SR - 1)) * stride);
+            const size_t b = (size_t)j * stepBytes;
+            return rsrc(gbase + firstRow - (b < lastBack ? b : lastBack));
         };
         auto loadStep = [&](double (&buf)[SR], int j) {
             const __amdgpu_buffer_rsrc_t p = rows(j);
@@ -417,6 +439,7 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             const int rd = diagRow(j + 1);
             const double p0 = du[rd], p1 = dw[rd];
             loadStep(ahead, j + 2);
+            settleStep(cur);
             const int slot = j & 1;
             const __amdgpu_buffer_rsrc_t out = rows(j);
 #pragma unroll
