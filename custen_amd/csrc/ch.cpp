@@ -47,6 +47,7 @@ struct CHCoeffs {
     double wlin[25];
     double wnl[9];
     std::vector<double> fac;  // ds, dl, d, du, dw, inv1, inv2
+    int constFwd = 0, constBwd = 0;  // chConstantRuns(fac)
 };
 
 // Validation and the constants of the scheme (everything but the factors).
@@ -87,6 +88,7 @@ bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
     if (!chConstants(p, co, why)) return false;
     co.fac.assign((size_t)7 * co.m, 0.0);
     chFactor(co.fac.data(), co.m, co.a, co.b, co.c, co.d, co.e, co.omega);
+    chConstantRuns(co.fac.data(), co.m, co.constFwd, co.constBwd);
     return true;
 }
 
@@ -377,6 +379,8 @@ private:
             for (int k = 0; k < 9; k++) d.wnl[k] = r->co.wnl[k];
             d.n = r->co.n;
             d.admitStep = r->admitStep;
+            d.constFwd1 = r->co.constFwd + 1;
+            d.constBwd1 = r->co.constBwd + 1;
             d.tileBase = tiles;
             d.colBase = cols;
             d.trBase = trs;

@@ -34,6 +34,10 @@ struct CHDesc {
     int tileBase;       // first rhs/final tile of this request in the launch
     int colBase;        // first column-solve block
     int trBase;         // first transpose tile
+    int constFwd1;      // 1 + first row of the forward factors' constant run
+    int constBwd1;      // 1 + first row of the backward factors' constant run
+                        // (chConstantRuns); 0 = not known: every row is read
+                        // from the factors
 };
 
 // Tile sizes of the work decomposition (host and device agree on these).

@@ -314,6 +314,16 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, off[k], 0, 0);
     };
     const int nFull = (m - 2) / SR;  // whole steps of each sweep
+    // The factors converge: from row cfw on (forward: ds, dl, d) and from row
+    // cbw to m-3 (backward: du, dw) every row's are bitwise the same.  Steps
+    // inside those runs take them as uniform constants and do without the
+    // ring; they run as a loop of their own after (forward) or before
+    // (backward) the steps that read the ring.
+    const int cfw = D.constFwd1 > 0 ? D.constFwd1 - 1 : m;
+    const int cbw = D.constBwd1 > 0 ? D.constBwd1 - 1 : m;
+    const int kf = min(cfw, m - 1), kb = min(cbw, m - 3);
+    const double k0 = ds[kf], k1 = dl[kf], k2 = dd[kf], ky = divRecip(k2);
+    const double ku = du[kb], kw = dw[kb];
 
     // ---- forward substitution: rows 0 and 1, then rows 2 .. m-1 in steps
     double b2 = at(0) / dd[0];
@@ -381,21 +391,67 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             }
             stageDiag(slot ^ 1, p0, p1, p2);
         };
+        // a step inside the constant run
+        auto stepK = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+            loadStep(ahead, j + 2);
+            settleStep(cur);
+            const __amdgpu_buffer_rsrc_t out = rows(j);
+            double nmin = 0x1p900, nmax = 0x1p-900;
+            const double b2s = b2, b1s = b1;
+#pragma unroll
+            for (int k = 0; k < SR; k++) {
+                const double t = cur[k] - k0 * b2;
+                const double num = t - k1 * b1;
+                const double v = divByRecip(num, k2, ky);
+                nmin = __builtin_fmin(nmin, __builtin_fabs(num));
+                nmax = __builtin_fmax(nmax, __builtin_fabs(num));
+                store(out, k, v);
+                b2 = b1;
+                b1 = v;
+            }
+            if (__builtin_expect(!(nmin >= 0x1p-900 && nmax <= 0x1p900 && ky != 0.0), 0)) {
+                b2 = b2s;
+                b1 = b1s;
+                for (int k = 0; k < SR; k++) {
+                    const double t = cur[k] - k0 * b2;
+                    const double v = (t - k1 * b1) / k2;
+                    store(out, k, v);
+                    b2 = b1;
+                    b1 = v;
+                }
+            }
+        };
+        // steps [0, jc) read the ring, steps [jc, nFull) start at or after cfw
+        const int jc = min((cfw - 2 + SR - 1) / SR, nFull);
         double bufA[SR], bufB[SR], bufC[SR];
-        {
-            const int r0 = diagRow(0);
-            stageDiag(0, ds[r0], dl[r0], dd[r0]);
+        if (jc > 0) {
+            {
+                const int r0 = diagRow(0);
+                stageDiag(0, ds[r0], dl[r0], dd[r0]);
+            }
+            loadStep(bufA, 0);
+            loadStep(bufB, 1);
+            int j = 0;
+            for (; j + 3 <= jc; j += 3) {
+                step(j, bufA, bufC);
+                step(j + 1, bufB, bufA);
+                step(j + 2, bufC, bufB);
+            }
+            if (j < jc) step(j++, bufA, bufC);
+            if (j < jc) step(j++, bufB, bufA);
         }
-        loadStep(bufA, 0);
-        loadStep(bufB, 1);
-        int j = 0;
-        for (; j + 3 <= nFull; j += 3) {
-            step(j, bufA, bufC);
-            step(j + 1, bufB, bufA);
-            step(j + 2, bufC, bufB);
+        if (jc < nFull) {
+            loadStep(bufA, jc);
+            loadStep(bufB, jc + 1);
+            int j = jc;
+            for (; j + 3 <= nFull; j += 3) {
+                stepK(j, bufA, bufC);
+                stepK(j + 1, bufB, bufA);
+                stepK(j + 2, bufC, bufB);
+            }
+            if (j < nFull) stepK(j++, bufA, bufC);
+            if (j < nFull) stepK(j++, bufB, bufA);
         }
-        if (j < nFull) step(j++, bufA, bufC);
-        if (j < nFull) step(j++, bufB, bufA);
     }
     // rows left after the whole steps, one at a time
     for (int i = 2 + nFull * SR; i < m; i++) {
@@ -435,12 +491,16 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             sd[slot][1][lane] = p1;
         };
         auto diagRow = [&](int j) { return max(m - 3 - j * SR - min(lane, SR - 1), 0); };
+        // steps [0, jcb) lie in the constant run (their lowest row is at or
+        // above cbw), steps [jcb, nFull) read the ring
+        const int lowest0 = m - 3 - (SR - 1);
+        const int jcb = lowest0 >= cbw ? min((lowest0 - cbw) / SR + 1, nFull) : 0;
         auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
             const int rd = diagRow(j + 1);
             const double p0 = du[rd], p1 = dw[rd];
             loadStep(ahead, j + 2);
             settleStep(cur);
-            const int slot = j & 1;
+            const int slot = (j - jcb) & 1;
             const __amdgpu_buffer_rsrc_t out = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) {
@@ -452,21 +512,48 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             }
             stageDiag(slot ^ 1, p0, p1);
         };
+        auto stepK = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+            loadStep(ahead, j + 2);
+            settleStep(cur);
+            const __amdgpu_buffer_rsrc_t out = rows(j);
+#pragma unroll
+            for (int k = 0; k < SR; k++) {
+                const double t = cur[k] - ku * a0;
+                const double v = t - kw * a1;
+                store(out, SR - 1 - k, v);
+                a1 = a0;
+                a0 = v;
+            }
+        };
         double bufA[SR], bufB[SR], bufC[SR];
-        {
-            const int r0 = diagRow(0);
-            stageDiag(0, du[r0], dw[r0]);
+        if (jcb > 0) {
+            loadStep(bufA, 0);
+            loadStep(bufB, 1);
+            int j = 0;
+            for (; j + 3 <= jcb; j += 3) {
+                stepK(j, bufA, bufC);
+                stepK(j + 1, bufB, bufA);
+                stepK(j + 2, bufC, bufB);
+            }
+            if (j < jcb) stepK(j++, bufA, bufC);
+            if (j < jcb) stepK(j++, bufB, bufA);
         }
-        loadStep(bufA, 0);
-        loadStep(bufB, 1);
-        int j = 0;
-        for (; j + 3 <= nFull; j += 3) {
-            step(j, bufA, bufC);
-            step(j + 1, bufB, bufA);
-            step(j + 2, bufC, bufB);
+        if (jcb < nFull) {
+            {
+                const int r0 = diagRow(jcb);
+                stageDiag(0, du[r0], dw[r0]);
+            }
+            loadStep(bufA, jcb);
+            loadStep(bufB, jcb + 1);
+            int j = jcb;
+            for (; j + 3 <= nFull; j += 3) {
+                step(j, bufA, bufC);
+                step(j + 1, bufB, bufA);
+                step(j + 2, bufC, bufB);
+            }
+            if (j < nFull) step(j++, bufA, bufC);
+            if (j < nFull) step(j++, bufB, bufA);
         }
-        if (j < nFull) step(j++, bufA, bufC);
-        if (j < nFull) step(j++, bufB, bufA);
     }
     for (int i = m - 3 - nFull * SR; i >= 0; i--) {
         const double t = at(i) - du[i] * a0;

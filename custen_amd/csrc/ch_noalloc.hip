@@ -79,6 +79,10 @@ __global__ __launch_bounds__(256) void setupKernel(CHDesc d, double cDiag, doubl
     const int m = d.n - 2;
     if (threadIdx.x == 0) {
         chFactor(lf, m, d.a, d.b, cDiag, d.d, d.e, d.omega);
+        int cf = 0, cb = 0;
+        chConstantRuns(lf, m, cf, cb);
+        d.constFwd1 = cf + 1;
+        d.constBwd1 = cb + 1;
         *out = d;
     }
     __syncthreads();

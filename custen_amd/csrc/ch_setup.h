@@ -65,6 +65,30 @@ CUSTEN_CH_HD inline void chPentSolve(const double* __restrict__ ds,
     for (int i = m - 3; i >= 0; i--) b[i] = b[i] - du[i] * b[i + 1] - dw[i] * b[i + 2];
 }
 
+// The factored rows of a constant-coefficient matrix converge: from some row
+// on they repeat bitwise.  fwd = the first row from which ds, dl, d equal
+// row m-1's (the rows the forward substitution reads up to its last row);
+// bwd = the first row from which du, dw equal row m-3's (the last row the
+// backward substitution's steps read).  When the factors never settle (a
+// two-value cycle, say) the runs are the last row alone.
+CUSTEN_CH_HD inline void chConstantRuns(const double* fac, int m, int& fwd, int& bwd)
+{
+    const double* ds = fac;
+    const double* dl = ds + m;
+    const double* dd = dl + m;
+    const double* du = dd + m;
+    const double* dw = du + m;
+    // bitwise: +0.0 and -0.0 are told apart, a NaN matches its own bits
+    auto bits = [](double v) { return __builtin_bit_cast(unsigned long long, v); };
+    const int lf = m - 1, lb = m - 3;
+    fwd = lf;
+    while (fwd > 0 && bits(ds[fwd - 1]) == bits(ds[lf]) && bits(dl[fwd - 1]) == bits(dl[lf]) &&
+           bits(dd[fwd - 1]) == bits(dd[lf]))
+        fwd--;
+    bwd = lb;
+    while (bwd > 0 && bits(du[bwd - 1]) == bits(du[lb]) && bits(dw[bwd - 1]) == bits(dw[lb])) bwd--;
+}
+
 // fac: 7 * m doubles, laid out ds, dl, d, du, dw, inv1, inv2; every entry is
 // written here (m = n - 2 >= 6).
 CUSTEN_CH_HD inline void chFactor(double* fac, int m, double a, double b, double c, double d,
