@@ -40,6 +40,26 @@ def test_solver_matches_oracle_config_sizes(gpu, n, steps, mode):
     test_solver_matches_oracle(gpu, n, steps, mode)
 
 
+@pytest.mark.parametrize("n,gamma", [(130, 1e-5), (130, 6.3e-4), (130, 4.24e-3), (200, 1e-4),
+                                     (600, 1e-6)])
+def test_solver_matches_oracle_coefficients(gpu, n, gamma):
+    """Coefficients whose factors settle early or never.  The column solve
+    takes the rows of the factors' constant runs as uniform constants; with
+    these coefficients the runs start at rows 8, 19, 127 (none: the last row
+    alone), 16 and 12, so a sweep has one step that reads the factors and the
+    rest constant, or every step reading the factors."""
+    c0 = ic(n, 3)
+    s = api.CHSolver(api.ch_params(n, gamma=gamma), c0)
+    s.run(3)
+    c, co = s.field()
+    s.close()
+    ref = oracle.CH(n, gamma=gamma)
+    rc, rco = c0.ravel().copy(), c0.ravel().copy()
+    ref.step(rc, rco, steps=3)
+    assert mismatches(c, rc.reshape(n, n)) == 0
+    assert mismatches(co, rco.reshape(n, n)) == 0
+
+
 @pytest.mark.parametrize("n", [256, 2048])
 def test_mass_conservation_200_steps(gpu, n):
     c0 = ic(n, 1)
