@@ -48,6 +48,7 @@ struct CHCoeffs {
     double wnl[9];
     std::vector<double> fac;  // ds, dl, d, du, dw, inv1, inv2
     int constFwd = 0, constBwd = 0;  // chConstantRuns(fac)
+    int fastDiv = 0;                 // chDivCertified(pivot of row constFwd)
 };
 
 // Validation and the constants of the scheme (everything but the factors).
@@ -89,6 +90,7 @@ bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
     co.fac.assign((size_t)7 * co.m, 0.0);
     chFactor(co.fac.data(), co.m, co.a, co.b, co.c, co.d, co.e, co.omega);
     chConstantRuns(co.fac.data(), co.m, co.constFwd, co.constBwd);
+    co.fastDiv = chDivCertified(co.fac[(size_t)2 * co.m + co.constFwd]) ? 1 : 0;
     return true;
 }
 
@@ -381,6 +383,7 @@ private:
             d.admitStep = r->admitStep;
             d.constFwd1 = r->co.constFwd + 1;
             d.constBwd1 = r->co.constBwd + 1;
+            d.fastDiv = r->co.fastDiv;
             d.tileBase = tiles;
             d.colBase = cols;
             d.trBase = trs;

@@ -38,6 +38,8 @@ struct CHDesc {
     int constBwd1;      // 1 + first row of the backward factors' constant run
                         // (chConstantRuns); 0 = not known: every row is read
                         // from the factors
+    int fastDiv;        // 1: the forward run's pivot passes chDivCertified, and
+                        // its steps divide in two operations
 };
 
 // Tile sizes of the work decomposition (host and device agree on these).

@@ -18,6 +18,8 @@
 // stencil sums are explicit fma chains (the cuSten order).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ch_kernels.h"
 
 #pragma clang fp contract(off)
@@ -337,8 +339,8 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             const size_t o = 2 * stride + (size_t)j * stepBytes;
             return rsrc(gbase + (o < lastRow ? o : lastRow));
         };
-        auto loadStep = [&](double (&buf)[SR], int j) {
-            const __amdgpu_buffer_rsrc_t p = rows(j);
+        auto loadStep = [&](double (&buf)[SR], __amdgpu_buffer_rsrc_t& p, int j) {
+            p = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) buf[k] = load(p, k);
         };
@@ -355,13 +357,13 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         auto diagRow = [&](int j) { return min(2 + j * SR + min(lane, SR - 1), m - 1); };
         // step j: diagonals of step j+1 and the column rows of step j+2 are
         // loaded, step j is computed from `cur` and ring slot j & 1
-        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR],
+                        const __amdgpu_buffer_rsrc_t& out, __amdgpu_buffer_rsrc_t& outAhead) {
             const int rd = diagRow(j + 1);
             const double p0 = ds[rd], p1 = dl[rd], p2 = dd[rd];
-            loadStep(ahead, j + 2);
+            loadStep(ahead, outAhead, j + 2);
             settleStep(cur);
             const int slot = j & 1;
-            const __amdgpu_buffer_rsrc_t out = rows(j);
             double nmin = 0x1p900, nmax = 0x1p-900;
             const double b2s = b2, b1s = b1;
 #pragma unroll
@@ -391,18 +393,23 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             }
             stageDiag(slot ^ 1, p0, p1, p2);
         };
-        // a step inside the constant run
-        auto stepK = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
-            loadStep(ahead, j + 2);
+        // a step inside the constant run; `fast`: the pivot is certified for
+        // the two-operation division q = fma(x, ky, RN(x kz)) = RN(x / k2)
+        // (2^-900 <= |x| <= 2^900), kz the low part of 1/k2
+        double kz = 0.0;
+        auto stepK = [&](auto fast, int j, const double (&cur)[SR], double (&ahead)[SR],
+                        const __amdgpu_buffer_rsrc_t& out, __amdgpu_buffer_rsrc_t& outAhead) {
+            loadStep(ahead, outAhead, j + 2);
             settleStep(cur);
-            const __amdgpu_buffer_rsrc_t out = rows(j);
             double nmin = 0x1p900, nmax = 0x1p-900;
             const double b2s = b2, b1s = b1;
 #pragma unroll
             for (int k = 0; k < SR; k++) {
                 const double t = cur[k] - k0 * b2;
                 const double num = t - k1 * b1;
-                const double v = divByRecip(num, k2, ky);
+                double v;
+                if constexpr (decltype(fast)::value) v = __builtin_fma(num, ky, num * kz);
+                else v = divByRecip(num, k2, ky);
                 nmin = __builtin_fmin(nmin, __builtin_fabs(num));
                 nmax = __builtin_fmax(nmax, __builtin_fabs(num));
                 store(out, k, v);
@@ -424,33 +431,42 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         // steps [0, jc) read the ring, steps [jc, nFull) start at or after cfw
         const int jc = min((cfw - 2 + SR - 1) / SR, nFull);
         double bufA[SR], bufB[SR], bufC[SR];
+        __amdgpu_buffer_rsrc_t rA, rB, rC;  // the rows of the steps in bufA, bufB, bufC
         if (jc > 0) {
             {
                 const int r0 = diagRow(0);
                 stageDiag(0, ds[r0], dl[r0], dd[r0]);
             }
-            loadStep(bufA, 0);
-            loadStep(bufB, 1);
+            loadStep(bufA, rA, 0);
+            loadStep(bufB, rB, 1);
             int j = 0;
             for (; j + 3 <= jc; j += 3) {
-                step(j, bufA, bufC);
-                step(j + 1, bufB, bufA);
-                step(j + 2, bufC, bufB);
+                step(j, bufA, bufC, rA, rC);
+                step(j + 1, bufB, bufA, rB, rA);
+                step(j + 2, bufC, bufB, rC, rB);
             }
-            if (j < jc) step(j++, bufA, bufC);
-            if (j < jc) step(j++, bufB, bufA);
+            if (j < jc) step(j++, bufA, bufC, rA, rC);
+            if (j < jc) step(j++, bufB, bufA, rB, rA);
         }
-        if (jc < nFull) {
-            loadStep(bufA, jc);
-            loadStep(bufB, jc + 1);
+        auto constSteps = [&](auto fast) {
+            loadStep(bufA, rA, jc);
+            loadStep(bufB, rB, jc + 1);
             int j = jc;
             for (; j + 3 <= nFull; j += 3) {
-                stepK(j, bufA, bufC);
-                stepK(j + 1, bufB, bufA);
-                stepK(j + 2, bufC, bufB);
+                stepK(fast, j, bufA, bufC, rA, rC);
+                stepK(fast, j + 1, bufB, bufA, rB, rA);
+                stepK(fast, j + 2, bufC, bufB, rC, rB);
             }
-            if (j < nFull) stepK(j++, bufA, bufC);
-            if (j < nFull) stepK(j++, bufB, bufA);
+            if (j < nFull) stepK(fast, j++, bufA, bufC, rA, rC);
+            if (j < nFull) stepK(fast, j++, bufB, bufA, rB, rA);
+        };
+        if (jc < nFull) {
+            if (D.fastDiv) {
+                kz = __builtin_fma(-k2, ky, 1.0) / k2;
+                constSteps(std::true_type{});
+            } else {
+                constSteps(std::false_type{});
+            }
         }
     }
     // rows left after the whole steps, one at a time
@@ -481,8 +497,8 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             const size_t b = (size_t)j * stepBytes;
             return rsrc(gbase + firstRow - (b < lastBack ? b : lastBack));
         };
-        auto loadStep = [&](double (&buf)[SR], int j) {
-            const __amdgpu_buffer_rsrc_t p = rows(j);
+        auto loadStep = [&](double (&buf)[SR], __amdgpu_buffer_rsrc_t& p, int j) {
+            p = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) buf[k] = load(p, SR - 1 - k);
         };
@@ -495,13 +511,13 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
         // above cbw), steps [jcb, nFull) read the ring
         const int lowest0 = m - 3 - (SR - 1);
         const int jcb = lowest0 >= cbw ? min((lowest0 - cbw) / SR + 1, nFull) : 0;
-        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
+        auto step = [&](int j, const double (&cur)[SR], double (&ahead)[SR],
+                        const __amdgpu_buffer_rsrc_t& out, __amdgpu_buffer_rsrc_t& outAhead) {
             const int rd = diagRow(j + 1);
             const double p0 = du[rd], p1 = dw[rd];
-            loadStep(ahead, j + 2);
+            loadStep(ahead, outAhead, j + 2);
             settleStep(cur);
             const int slot = (j - jcb) & 1;
-            const __amdgpu_buffer_rsrc_t out = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) {
                 const double t = cur[k] - sd[slot][0][k] * a0;
@@ -512,10 +528,10 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             }
             stageDiag(slot ^ 1, p0, p1);
         };
-        auto stepK = [&](int j, const double (&cur)[SR], double (&ahead)[SR]) {
-            loadStep(ahead, j + 2);
+        auto stepK = [&](int j, const double (&cur)[SR], double (&ahead)[SR],
+                        const __amdgpu_buffer_rsrc_t& out, __amdgpu_buffer_rsrc_t& outAhead) {
+            loadStep(ahead, outAhead, j + 2);
             settleStep(cur);
-            const __amdgpu_buffer_rsrc_t out = rows(j);
 #pragma unroll
             for (int k = 0; k < SR; k++) {
                 const double t = cur[k] - ku * a0;
@@ -526,33 +542,34 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
             }
         };
         double bufA[SR], bufB[SR], bufC[SR];
+        __amdgpu_buffer_rsrc_t rA, rB, rC;  // the rows of the steps in bufA, bufB, bufC
         if (jcb > 0) {
-            loadStep(bufA, 0);
-            loadStep(bufB, 1);
+            loadStep(bufA, rA, 0);
+            loadStep(bufB, rB, 1);
             int j = 0;
             for (; j + 3 <= jcb; j += 3) {
-                stepK(j, bufA, bufC);
-                stepK(j + 1, bufB, bufA);
-                stepK(j + 2, bufC, bufB);
+                stepK(j, bufA, bufC, rA, rC);
+                stepK(j + 1, bufB, bufA, rB, rA);
+                stepK(j + 2, bufC, bufB, rC, rB);
             }
-            if (j < jcb) stepK(j++, bufA, bufC);
-            if (j < jcb) stepK(j++, bufB, bufA);
+            if (j < jcb) stepK(j++, bufA, bufC, rA, rC);
+            if (j < jcb) stepK(j++, bufB, bufA, rB, rA);
         }
         if (jcb < nFull) {
             {
                 const int r0 = diagRow(jcb);
                 stageDiag(0, du[r0], dw[r0]);
             }
-            loadStep(bufA, jcb);
-            loadStep(bufB, jcb + 1);
+            loadStep(bufA, rA, jcb);
+            loadStep(bufB, rB, jcb + 1);
             int j = jcb;
             for (; j + 3 <= nFull; j += 3) {
-                step(j, bufA, bufC);
-                step(j + 1, bufB, bufA);
-                step(j + 2, bufC, bufB);
+                step(j, bufA, bufC, rA, rC);
+                step(j + 1, bufB, bufA, rB, rA);
+                step(j + 2, bufC, bufB, rC, rB);
             }
-            if (j < nFull) step(j++, bufA, bufC);
-            if (j < nFull) step(j++, bufB, bufA);
+            if (j < nFull) step(j++, bufA, bufC, rA, rC);
+            if (j < nFull) step(j++, bufB, bufA, rB, rA);
         }
     }
     for (int i = m - 3 - nFull * SR; i >= 0; i--) {

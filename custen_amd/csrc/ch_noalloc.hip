@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256) void setupKernel(CHDesc d, double cDiag, doubl
         chConstantRuns(lf, m, cf, cb);
         d.constFwd1 = cf + 1;
         d.constBwd1 = cb + 1;
+        d.fastDiv = chDivCertified(lf[2 * m + cf]) ? 1 : 0;
         *out = d;
     }
     __syncthreads();

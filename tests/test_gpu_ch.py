@@ -41,13 +41,15 @@ def test_solver_matches_oracle_config_sizes(gpu, n, steps, mode):
 
 
 @pytest.mark.parametrize("n,gamma", [(130, 1e-5), (130, 6.3e-4), (130, 4.24e-3), (200, 1e-4),
-                                     (600, 1e-6)])
+                                     (600, 1e-6), (200, 8.7155907323626728e-05)])
 def test_solver_matches_oracle_coefficients(gpu, n, gamma):
     """Coefficients whose factors settle early or never.  The column solve
     takes the rows of the factors' constant runs as uniform constants; with
     these coefficients the runs start at rows 8, 19, 127 (none: the last row
-    alone), 16 and 12, so a sweep has one step that reads the factors and the
-    rest constant, or every step reading the factors."""
+    alone), 16, 12 and 16, so a sweep has one step that reads the factors and
+    the rest constant, or every step reading the factors.  The last pivot is
+    one that the two-operation division check refuses, so its constant steps
+    take the three-correction division."""
     c0 = ic(n, 3)
     s = api.CHSolver(api.ch_params(n, gamma=gamma), c0)
     s.run(3)
