@@ -97,7 +97,7 @@ bool chSetup(const custenCHParams& p, CHCoeffs& co, std::string& why)
 // Device buffers of one request, recycled by grid size.
 struct CHBuffers {
     int n = 0;
-    double* f[2] = {nullptr, nullptr};
+    double* f[3] = {nullptr, nullptr, nullptr};  // field buffers, used in turn
     double* X = nullptr;
     double* Y = nullptr;
     double* fac = nullptr;
@@ -106,7 +106,7 @@ struct CHBuffers {
 size_t footprintBytes(int n)
 {
     const size_t nn = (size_t)n * n;
-    return 4 * nn * sizeof(double) + (size_t)7 * (n - 2) * sizeof(double);
+    return 5 * nn * sizeof(double) + (size_t)7 * (n - 2) * sizeof(double);
 }
 
 class BufferPool {
@@ -132,6 +132,7 @@ public:
         hipError_t e;
         if ((e = hipMalloc(&out.f[0], nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.f[1], nn * sizeof(double))) != hipSuccess ||
+            (e = hipMalloc(&out.f[2], nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.X, nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.Y, nn * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&out.fac, (size_t)7 * (n - 2) * sizeof(double))) != hipSuccess) {
@@ -154,9 +155,9 @@ public:
 private:
     static void release(CHBuffers& b)
     {
-        for (double* p : {b.f[0], b.f[1], b.X, b.Y, b.fac})
+        for (double* p : {b.f[0], b.f[1], b.f[2], b.X, b.Y, b.fac})
             if (p) (void)hipFree(p);
-        b.f[0] = b.f[1] = b.X = b.Y = b.fac = nullptr;
+        b.f[0] = b.f[1] = b.f[2] = b.X = b.Y = b.fac = nullptr;
     }
     size_t keep_;
     size_t held_ = 0;
@@ -181,14 +182,15 @@ struct CHRequest {
 };
 
 // Writes a request's fields and factors into its device buffers: c0 into
-// f[0]; cOld0 into f[1], or a copy of c0 when cOld0 is null.
+// f[0] (the current field at the request's first step) and cOld0 into f[2]
+// (the previous one), or a copy of c0 when cOld0 is null.
 hipError_t loadRequest(const CHRequest* r, const double* c0, const double* cOld0, hipStream_t st)
 {
     const size_t bytes = (size_t)r->co.n * r->co.n * sizeof(double);
     hipError_t e;
     if ((e = hipMemcpyAsync(r->buf.f[0], c0, bytes, hipMemcpyDefault, st)) != hipSuccess) return e;
-    if (cOld0) e = hipMemcpyAsync(r->buf.f[1], cOld0, bytes, hipMemcpyDefault, st);
-    else e = hipMemcpyAsync(r->buf.f[1], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st);
+    if (cOld0) e = hipMemcpyAsync(r->buf.f[2], cOld0, bytes, hipMemcpyDefault, st);
+    else e = hipMemcpyAsync(r->buf.f[2], r->buf.f[0], bytes, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(r->buf.fac, r->co.fac.data(), r->co.fac.size() * sizeof(double),
                           hipMemcpyHostToDevice, st);
@@ -234,11 +236,13 @@ public:
 
     const std::vector<CHRequest*>& active() const { return active_; }
 
-    // Current field of a resident request.
-    double* current(const CHRequest* r) const { return r->buf.f[(step_ - r->admitStep) & 1]; }
+    // Current and previous field of a resident request: the three buffers
+    // are used in turn, each step writing the new field to the one after
+    // the current.
+    double* current(const CHRequest* r) const { return r->buf.f[(step_ - r->admitStep) % 3]; }
     double* previous(const CHRequest* r) const
     {
-        return r->buf.f[((step_ - r->admitStep) & 1) ^ 1];
+        return r->buf.f[(step_ - r->admitStep + 2) % 3];
     }
 
     // A step done outside the engine (the plain path) that kept the field in
@@ -250,35 +254,24 @@ public:
         dirty_ = true;
     }
 
-    // One step of all resident requests.  plain = true synchronises the stream
-    // after every launch.
-    hipError_t stepAll(hipStream_t st, bool plain)
+    // k steps of all resident requests, launched directly on `st`.
+    hipError_t run(hipStream_t st, int k)
     {
-        if (active_.empty()) return hipSuccess;
+        if (active_.empty() || k <= 0) return hipSuccess;
         hipError_t e;
         if (dirty_ && (e = upload(st)) != hipSuccess) return e;
-        auto sync = [&]() { return plain ? hipStreamSynchronize(st) : hipSuccess; };
-        if ((e = chLaunchRhs(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
-        if ((e = chLaunchFinal(dDescs_, count_, tiles_, step_, st)) != hipSuccess) return e;
-        if ((e = sync()) != hipSuccess) return e;
-        step_++;
+        if ((e = launchSteps(st, step_, k)) != hipSuccess) return e;
+        step_ += k;
         return hipSuccess;
     }
 
     // k steps of all resident requests as one launch: the step kernels of the
     // k steps are captured once into a HIP graph and replayed while the
     // resident set (and so the descriptors) and k stay the same.  The
-    // kernels use a step number only through its parity relative to each
-    // request's admission, so a graph serves every batch whose first step has
-    // the parity it was captured at.  Replaying one graph costs the host one
-    // launch instead of 5 k.  Consecutive batches alternate between two
+    // kernels use a step number only modulo 3 relative to each request's
+    // admission (which field buffer is current), so a graph serves every
+    // batch whose first step has the residue it was captured at.  Replaying
+    // one graph costs the host one launch instead of 4 k + 1.  Consecutive batches alternate between two
     // graphs: launching a graph whose previous launch is still running makes
     // the host wait for it.
     hipError_t stepBatch(hipStream_t st, int k)
@@ -287,14 +280,12 @@ public:
         hipError_t e;
         if (dirty_ && (e = upload(st)) != hipSuccess) return e;
         BatchGraph& g = graphs_[launches_ & 1];
-        if (!g.exec || g.k != k || g.version != version_ || g.parity != (step_ & 1)) {
+        if (!g.exec || g.k != k || g.version != version_ || g.parity != step_ % 3) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g.exec = nullptr;
             if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return e;
-            const int s0 = step_;
-            hipError_t le = hipSuccess;
-            for (int i = 0; i < k && le == hipSuccess; i++) le = launchStep(st, s0 + i);
+            const hipError_t le = launchSteps(st, step_, k);
             hipGraph_t graph = nullptr;
             e = hipStreamEndCapture(st, &graph);
             if (le != hipSuccess) e = le;
@@ -306,7 +297,7 @@ public:
             }
             g.k = k;
             g.version = version_;
-            g.parity = step_ & 1;
+            g.parity = step_ % 3;
         }
         if ((e = hipGraphLaunch(g.exec, st)) != hipSuccess) return e;
         launches_++;
@@ -329,14 +320,23 @@ private:
         }
     }
 
-    hipError_t launchStep(hipStream_t st, int step)
+    // Steps s0 .. s0 + k - 1: the rhs of the first, then per step the two
+    // solves and the transpose, and the final phase fused with the next
+    // step's rhs (the last step's final alone).
+    hipError_t launchSteps(hipStream_t st, int s0, int k)
     {
         hipError_t e;
-        if ((e = chLaunchRhs(dDescs_, count_, tiles_, step, st)) != hipSuccess) return e;
-        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
-        if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, st)) != hipSuccess) return e;
-        if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
-        return chLaunchFinal(dDescs_, count_, tiles_, step, st);
+        if ((e = chLaunchRhs(dDescs_, count_, tiles_, s0, st)) != hipSuccess) return e;
+        for (int i = 0; i < k; i++) {
+            const int s = s0 + i;
+            if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 0, st)) != hipSuccess) return e;
+            if ((e = chLaunchTranspose(dDescs_, count_, trTiles_, st)) != hipSuccess) return e;
+            if ((e = chLaunchColSolve(dDescs_, count_, colBlocks_, 1, st)) != hipSuccess) return e;
+            e = i + 1 < k ? chLaunchFinalRhs(dDescs_, count_, tiles_, s, st)
+                          : chLaunchFinal(dDescs_, count_, tiles_, s, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     }
 
     hipError_t upload(hipStream_t st)
@@ -369,6 +369,8 @@ private:
             std::memset(&d, 0, sizeof(d));
             d.f[0] = r->buf.f[0];
             d.f[1] = r->buf.f[1];
+            d.f[2] = r->buf.f[2];
+            d.ring = 3;
             d.X = r->buf.X;
             d.Y = r->buf.Y;
             d.fac = r->buf.fac;
@@ -420,7 +422,7 @@ private:
     struct BatchGraph {
         hipGraphExec_t exec = nullptr;
         int k = 0;
-        int parity = 0;  // of the batch's first step
+        int parity = 0;  // the batch's first step modulo 3
         uint64_t version = 0;
     };
     BatchGraph graphs_[2];
@@ -713,12 +715,10 @@ extern "C" int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream
     (void)hipSetDevice(ch->device);
     hipStream_t st = stream ? stream : ch->stream;
     if (mode == 0) {
-        for (int s = 0; s < steps; s++) {
-            hipError_t e = ch->engine->stepAll(st, false);
-            if (e != hipSuccess) {
-                setHipError(e, "custenCHRun");
-                return lastError();
-            }
+        hipError_t e = ch->engine->run(st, steps);
+        if (e != hipSuccess) {
+            setHipError(e, "custenCHRun");
+            return lastError();
         }
         ch->req.done += steps;
         return CUSTEN_SUCCESS;

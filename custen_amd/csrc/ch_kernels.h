@@ -19,9 +19,11 @@ namespace custen {
 // Per-request constants of the ADI scheme.  Everything a kernel needs for
 // one resident simulation; an array of these lives in device memory.
 struct CHDesc {
-    double* f[2];       // the two time levels; at engine step s the current field
-                        // is f[(s - admitStep) & 1] and the previous one the other;
-                        // the final phase writes the new field over the previous one
+    double* f[3];       // field buffers used in turn (the first `ring`): at engine
+                        // step s, with t = s - admitStep, the current field is
+                        // f[t % ring], the previous one f[(t + ring - 1) % ring],
+                        // and the step writes the new one to f[(t + 1) % ring]
+                        // (ring 2: over the previous field, in place)
     double* X;          // solution work, n*n (y-direction solves)
     double* Y;          // transposed right-hand side, n*n (x-direction solves)
     const double* fac;  // factored diagonals + inverses: ds, dl, d, du, dw, inv1, inv2 (7 * m)
@@ -30,6 +32,7 @@ struct CHDesc {
     double wlin[25];    // 5x5 biharmonic RHS weights
     double wnl[9];      // 3x3 Laplacian of the nonlinear term
     int n;
+    int ring;           // 2 or 3 field buffers; finalRhs needs 3
     int admitStep;      // engine step at which f[0] held the current field
     int tileBase;       // first rhs/final tile of this request in the launch
     int colBase;        // first column-solve block
@@ -61,11 +64,17 @@ CUSTEN_HD inline int chTrTiles(int n) { const int t = (n + kCHTr - 1) / kCHTr; r
 //   transpose  X = (Y with the column solveFull applied)^T
 //   colSolve   cyclic solve of every column of X          (which = 1)
 //   final      c_new = cBar + (X with the column solveFull applied)
+//   finalRhs   final of step `step` and rhs of step + 1 in one pass: c_new is
+//              computed over the tile and its halo from X, c and cOld, written
+//              to the third field buffer (ring 3 only), and the next
+//              right-hand side is formed from it and c
 hipError_t chLaunchRhs(const CHDesc* descs, int count, int totalTiles, int step, hipStream_t st);
 hipError_t chLaunchTranspose(const CHDesc* descs, int count, int totalTiles, hipStream_t st);
 hipError_t chLaunchColSolve(const CHDesc* descs, int count, int totalColBlocks, int which,
                             hipStream_t st);
 hipError_t chLaunchFinal(const CHDesc* descs, int count, int totalTiles, int step, hipStream_t st);
+hipError_t chLaunchFinalRhs(const CHDesc* descs, int count, int totalTiles, int step,
+                            hipStream_t st);
 
 // Validates `p` and fills the constants of a descriptor: a, b, d, e, the two
 // weight stencils and n (pointers, omega and the work bases are left zero);

@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "ch_kernels.h"
+#include "stencil_tile.h"  // xcdRemap
 
 #pragma clang fp contract(off)
 
@@ -46,6 +47,9 @@ __device__ __forceinline__ int findRequest(const CHDesc* d, int count, int item,
     return lo;
 }
 
+// t modulo the number of field buffers (2 or 3), t >= 0
+__device__ __forceinline__ int ringIdx(int t, int ring) { return ring == 2 ? (t & 1) : (t % 3); }
+
 __device__ __forceinline__ int wrapN(int i, int n)
 {
     if (i < 0) return i + n;
@@ -60,19 +64,78 @@ constexpr int RX = kCHTileX, RY = kCHTileY;
 constexpr int CBW = RX + 4, CBH = RY + 4;  // cBar tile with a 2-point halo
 constexpr int QW = RX + 2, QH = RY + 2;    // u^3 - u tile with a 1-point halo
 
+// The right-hand side of a 64 x 16 tile from the staged cBar (cb, 2-point
+// halo) and u^3 - u (q, 1-point halo) and this thread's four values of
+// c - cOld (dcs, rows wave*4 .. wave*4+3 of column x0 + lane), written
+// transposed to Y.  The caller has synchronised after staging.  The result
+// goes through cb's storage, reused once every thread has read it.
+static_assert(RX * (RY + 1) <= CBH * CBW, "result tile must fit in the cBar tile");
+__device__ __forceinline__ void rhsTile(const CHDesc& D, double* cb, const double* q,
+                                        const double (&dcs)[4], int x0, int y0,
+                                        gdouble* __restrict__ Yo)
+{
+    double (*ot)[RY + 1] = reinterpret_cast<double (*)[RY + 1]>(cb);
+    const int n = D.n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double k23 = -(2.0 / 3.0);
+    // The four rows of this lane's column are summed together, weight by
+    // weight (each row's chain keeps the cuSten order: j outer, i inner), so
+    // each weight is fetched once for the four rows.  Points past the grid
+    // are computed from in-bounds LDS values and never stored.
+    const int ly0 = wave * 4;
+    double lin[4] = {0.0, 0.0, 0.0, 0.0}, nl[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const double w = D.wlin[j * 5 + i];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                lin[rr] = __builtin_fma(w, cb[(ly0 + rr + j) * CBW + lane + i], lin[rr]);
+        }
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double w = D.wnl[j * 3 + i];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                nl[rr] = __builtin_fma(w, q[(ly0 + rr + j) * QW + lane + i], nl[rr]);
+        }
+    double res[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const double t2 = k23 * dcs[rr];
+        const double t3 = t2 + nl[rr];
+        res[rr] = lin[rr] + t3;
+    }
+    __syncthreads();  // every thread has read cb
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) ot[lane][ly0 + rr] = res[rr];
+    __syncthreads();
+    // the tile goes out transposed (Y = X^T), 16 consecutive y per x: each
+    // store instruction of a wave writes four whole 128-byte segments
+#pragma unroll
+    for (int i = 0; i < RX * RY / 256; i++) {
+        const int idx = i * 256 + (int)threadIdx.x;
+        const int xl = idx / RY, yl = idx % RY;
+        if (x0 + xl < n && y0 + yl < n) Yo[(int64_t)(x0 + xl) * n + y0 + yl] = ot[xl][yl];
+    }
+}
+
 __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ descs, int count,
                                                  int step)
 {
-    __shared__ double cb[CBH * CBW];
+    __shared__ double cb[CBH * CBW];  // then the tile's result, [x][y]
     __shared__ double q[QH * QW];
-    __shared__ double ot[RX][RY + 1];  // the tile's result, [x][y]
-    const int item = blockIdx.x;
+    // neighbouring tiles on one XCD, so halo lines are re-read from its L2
+    const int item = dev::xcdRemap(blockIdx.x, gridDim.x);
     const int r = findRequest(descs, count, item, 0);
     const CHDesc& D = descs[r];
     const int n = D.n;
-    const int par = (step - D.admitStep) & 1;
-    const gdouble* __restrict__ c = (const gdouble*)D.f[par];
-    const gdouble* __restrict__ co = (const gdouble*)D.f[par ^ 1];
+    const int tl = step - D.admitStep, ring = D.ring;
+    const gdouble* __restrict__ c = (const gdouble*)D.f[ringIdx(tl, ring)];
+    const gdouble* __restrict__ co = (const gdouble*)D.f[ringIdx(tl + ring - 1, ring)];
     gdouble* __restrict__ Yo = (gdouble*)D.Y;
     const int local = item - D.tileBase;
     const int tilesX = (n + RX - 1) / RX;
@@ -121,49 +184,11 @@ __global__ __launch_bounds__(256) void rhsKernel(const CHDesc* __restrict__ desc
             q[(rr - 1) * QW + (cc - 1)] = u3 - cv[i];
         }
     }
+    double dcs[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) dcs[rr] = cOwn[rr] - oOwn[rr];
     __syncthreads();
-
-    const double k23 = -(2.0 / 3.0);
-    // The four rows of this lane's column are summed together, weight by
-    // weight (each row's chain keeps the cuSten order: j outer, i inner), so
-    // each weight is fetched once for the four rows.  Points past the grid
-    // are computed from in-bounds LDS values and never stored.
-    const int ly0 = wave * 4;
-    double lin[4] = {0.0, 0.0, 0.0, 0.0}, nl[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < 5; j++)
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const double w = D.wlin[j * 5 + i];
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-                lin[rr] = __builtin_fma(w, cb[(ly0 + rr + j) * CBW + lane + i], lin[rr]);
-        }
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double w = D.wnl[j * 3 + i];
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-                nl[rr] = __builtin_fma(w, q[(ly0 + rr + j) * QW + lane + i], nl[rr]);
-        }
-#pragma unroll
-    for (int rr = 0; rr < 4; rr++) {
-        const double t1 = cOwn[rr] - oOwn[rr];
-        const double t2 = k23 * t1;
-        const double t3 = t2 + nl[rr];
-        ot[lane][ly0 + rr] = lin[rr] + t3;
-    }
-    __syncthreads();
-    // the tile goes out transposed (Y = X^T), 16 consecutive y per x: each
-    // store instruction of a wave writes four whole 128-byte segments
-#pragma unroll
-    for (int i = 0; i < RX * RY / 256; i++) {
-        const int idx = i * 256 + (int)threadIdx.x;
-        const int xl = idx / RY, yl = idx % RY;
-        if (x0 + xl < n && y0 + yl < n) Yo[(int64_t)(x0 + xl) * n + y0 + yl] = ot[xl][yl];
-    }
+    rhsTile(D, cb, q, dcs, x0, y0, Yo);
 }
 
 // ---------------------------------------------------------------------------
@@ -589,7 +614,8 @@ __global__ __launch_bounds__(64, 2) void colSolveKernel(const CHDesc* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// final: column solveFull + findNew, written over the previous field
+// final: column solveFull + findNew, written to the next field buffer (with
+// two buffers: over the previous field, each point by the thread that read it)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void finalKernel(const CHDesc* __restrict__ descs, int count,
                                                    int step)
@@ -598,9 +624,11 @@ __global__ __launch_bounds__(256) void finalKernel(const CHDesc* __restrict__ de
     const int r = findRequest(descs, count, item, 0);
     const CHDesc& D = descs[r];
     const int n = D.n, m = n - 2;
-    const int par = (step - D.admitStep) & 1;
-    const gdouble* __restrict__ c = (const gdouble*)D.f[par];
-    gdouble* __restrict__ co = (gdouble*)D.f[par ^ 1];
+    const int tl = step - D.admitStep, ring = D.ring;
+    const gdouble* __restrict__ c = (const gdouble*)D.f[ringIdx(tl, ring)];
+    // co and out are the same buffer when ring == 2
+    const gdouble* co = (const gdouble*)D.f[ringIdx(tl + ring - 1, ring)];
+    gdouble* out = (gdouble*)D.f[ringIdx(tl + 1, ring)];
     const gdouble* __restrict__ X = (const gdouble*)D.X;
     const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
     const gdouble* inv2 = inv1 + m;
@@ -635,8 +663,102 @@ __global__ __launch_bounds__(256) void finalKernel(const CHDesc* __restrict__ de
         if (y < m) xf = xf - (i1[rr] * e2 + i2[rr] * e1);
         const double t = 2.0 * cv[rr];
         const double cBar = t - ov[rr];
-        co[(int64_t)y * n + x] = cBar + xf;
+        out[(int64_t)y * n + x] = cBar + xf;
     }
+}
+
+// ---------------------------------------------------------------------------
+// finalRhs: final of step `step` and rhs of step + 1 (three field buffers).
+// The new field is pointwise in X, c and cOld, so it is computed over the
+// tile and its 2-point halo from those (the halo points as their own tiles
+// compute them: the same operations on the same values), written to the
+// third buffer at the tile's points, and the next right-hand side is formed
+// from it and c without reading it back.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void finalRhsKernel(const CHDesc* __restrict__ descs, int count,
+                                                      int step)
+{
+    __shared__ double cb[CBH * CBW];  // then the tile's result, [x][y]
+    __shared__ double q[QH * QW];
+    __shared__ double dn[RY][RX];           // c_new - c at the tile's points
+    __shared__ double ce2[CBW], ce1[CBW];   // X rows n-2 and n-1 at the staged columns
+    __shared__ double ri1[CBH], ri2[CBH];   // inv1, inv2 at the staged rows
+    // neighbouring tiles on one XCD, so halo lines are re-read from its L2
+    const int item = dev::xcdRemap(blockIdx.x, gridDim.x);
+    const int r = findRequest(descs, count, item, 0);
+    const CHDesc& D = descs[r];
+    const int n = D.n, m = n - 2;
+    const int tl = step - D.admitStep, ring = D.ring;
+    const gdouble* __restrict__ c = (const gdouble*)D.f[ringIdx(tl, ring)];
+    const gdouble* __restrict__ co = (const gdouble*)D.f[ringIdx(tl + ring - 1, ring)];
+    gdouble* __restrict__ out = (gdouble*)D.f[ringIdx(tl + 1, ring)];
+    const gdouble* __restrict__ X = (const gdouble*)D.X;
+    gdouble* __restrict__ Yo = (gdouble*)D.Y;
+    const gdouble* inv1 = (const gdouble*)D.fac + 5 * m;
+    const gdouble* inv2 = inv1 + m;
+    const int local = item - D.tileBase;
+    const int tilesX = (n + RX - 1) / RX;
+    const int x0 = (local % tilesX) * RX;
+    const int y0 = (local / tilesX) * RY;
+    const int tid = (int)threadIdx.x;
+
+    // staged points as in rhsKernel: [-2, n + 1] wrapped, clamped past n + 1
+    constexpr int kStage = (CBH * CBW + 255) / 256;
+    double xv[kStage], cv[kStage], ov[kStage];
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int idx = min(tid + i * 256, CBH * CBW - 1);
+        const int rr = idx / CBW, cc = idx - rr * CBW;
+        const int gy = wrapN(min(y0 - 2 + rr, n + 1), n);
+        const int gx = wrapN(min(x0 - 2 + cc, n + 1), n);
+        const int64_t k = (int64_t)gy * n + gx;
+        xv[i] = X[k];
+        cv[i] = c[k];
+        ov[i] = co[k];
+    }
+    if (tid < CBW) {
+        const int gx = wrapN(min(x0 - 2 + tid, n + 1), n);
+        ce2[tid] = X[(int64_t)(n - 2) * n + gx];
+        ce1[tid] = X[(int64_t)(n - 1) * n + gx];
+    } else if (tid < CBW + CBH) {
+        const int rr = tid - CBW;
+        const int yi = min(wrapN(min(y0 - 2 + rr, n + 1), n), m - 1);
+        ri1[rr] = inv1[yi];
+        ri2[rr] = inv2[yi];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int idx = tid + i * 256;
+        if (idx >= CBH * CBW) break;
+        const int rr = idx / CBW, cc = idx - rr * CBW;
+        const int gy = wrapN(min(y0 - 2 + rr, n + 1), n);
+        // final (as finalKernel)
+        double xf = xv[i];
+        if (gy < m) xf = xf - (ri1[rr] * ce2[cc] + ri2[rr] * ce1[cc]);
+        const double t = 2.0 * cv[i];
+        const double cBar = t - ov[i];
+        const double cn = cBar + xf;
+        // staging of the next step's rhs (as rhsKernel, c -> cn, cOld -> c)
+        const double t2 = 2.0 * cn;
+        cb[idx] = t2 - cv[i];
+        if (rr >= 1 && rr < CBH - 1 && cc >= 1 && cc < CBW - 1) {
+            const double u2 = cn * cn;
+            const double u3 = u2 * cn;
+            q[(rr - 1) * QW + (cc - 1)] = u3 - cn;
+        }
+        if (rr >= 2 && rr < 2 + RY && cc >= 2 && cc < 2 + RX) {
+            const int y = y0 + rr - 2, x = x0 + cc - 2;
+            if (y < n && x < n) out[(int64_t)y * n + x] = cn;
+            dn[rr - 2][cc - 2] = cn - cv[i];
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    double dcs[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) dcs[rr] = dn[wave * 4 + rr][lane];
+    rhsTile(D, cb, q, dcs, x0, y0, Yo);
 }
 
 }  // namespace chdev
@@ -661,6 +783,15 @@ hipError_t chLaunchColSolve(const CHDesc* descs, int count, int totalColBlocks, 
     if (count <= 0 || totalColBlocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(chdev::colSolveKernel, dim3(totalColBlocks), dim3(kCHColBlock), 0, st,
                        descs, count, which);
+    return hipGetLastError();
+}
+
+hipError_t chLaunchFinalRhs(const CHDesc* descs, int count, int totalTiles, int step,
+                            hipStream_t st)
+{
+    if (count <= 0 || totalTiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(chdev::finalRhsKernel, dim3(totalTiles), dim3(256), 0, st, descs, count,
+                       step);
     return hipGetLastError();
 }
 

@@ -152,8 +152,12 @@ extern "C" int custenCHAdvanceNoAlloc(int deviceNum, const custenCHParams* param
 
     CHDesc* dDesc = reinterpret_cast<CHDesc*>(base);
     double* fac = base + kDescDoubles;
+    // two field buffers: each step writes the new field over the previous
+    // one (the final phase is pointwise), so no third n*n array is needed
     d.f[0] = c;
     d.f[1] = cOld;
+    d.f[2] = nullptr;
+    d.ring = 2;
     d.X = fac + kFacDoubles;
     d.Y = d.X + kFieldDoubles;
     d.fac = fac;

@@ -105,7 +105,8 @@ int custenCHServerResult(custenCHServer* s, int id, double* c);
 int custenCHAdvanceNoAlloc(int deviceNum, const custenCHParams* params, double* c, double* cOld,
                            int steps, hipStream_t stream);
 
-/* Bytes a request of grid size n keeps resident. */
+/* Bytes a request of grid size n keeps resident: five n*n fields (three time
+ * levels used in turn and two work fields) and the factored matrix. */
 size_t custenCHFootprint(int n);
 void custenCHServerDestroy(custenCHServer* s);
 

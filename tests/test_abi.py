@@ -106,7 +106,8 @@ def test_ch_validation_without_gpu():
     assert not L.custenCHCreate(0, ctypes.byref(p), c0.ctypes.data)
     assert L.custenGetLastError() == 1
     api.clear_error()
-    assert L.custenCHFootprint(64) == 4 * 64 * 64 * 8 + 7 * 62 * 8
+    # three field buffers used in turn, two work fields, the factors
+    assert L.custenCHFootprint(64) == 5 * 64 * 64 * 8 + 7 * 62 * 8
     assert not L.custenCHServerCreate(0, 0, 4)
     assert L.custenGetLastError() == 1
     api.clear_error()
