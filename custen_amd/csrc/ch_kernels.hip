@@ -9,7 +9,9 @@
 //   xsolve: cyclic pentadiagonal solve of every column of Y (pentSolve +
 //           solveEnd), X = Y^T with solveFull applied on the way
 //   ysolve: cyclic solve of every column of X (pentSolve + solveEnd)
-//   final : c_new = cBar + (X - (inv1 x~2 + inv2 x~1)), written over cOld
+//   final : c_new = cBar + (X - (inv1 x~2 + inv2 x~1)), written to the next
+//           field buffer; inside a run of steps it is fused with the next
+//           step's rhs (finalRhs)
 // Every solve is a thread-per-column sweep (coalesced across the wave) with a
 // block of row loads in flight ahead of the recurrence; the transposed store
 // of the right-hand side and the transpose after the x solve replace the

@@ -31,8 +31,9 @@ typedef struct {
 /* Single-simulation solver with resident fields.  c0: n*n initial condition
  * (host or device); cOld starts equal to c0 as in the reference. */
 custenCH* custenCHCreate(int deviceNum, const custenCHParams* params, const double* c0);
-/* Advance `steps` steps.  mode 0: the five fused step kernels are queued back
- * to back; mode 1: the reference's kernel sequence with a stream
+/* Advance `steps` steps.  mode 0: the fused step kernels are queued back to
+ * back (four per step: two column solves, the transpose, and the final phase
+ * fused with the next step's right-hand side); mode 1: the reference's kernel sequence with a stream
  * synchronisation after each launch (the reference's launch-and-synchronise loop, cuPentCahnADI.cu:528-596),
  * used as the plain per-grid baseline. */
 int custenCHRun(custenCH* ch, int steps, int mode, hipStream_t stream);
