@@ -264,6 +264,10 @@ __global__ __launch_bounds__(256) void transposeKernel(const CHDesc* __restrict_
 // clamped to a valid row and their values unused), so the wait counts the
 // compiler computes are exact and no wait drains the loads still in flight.
 // Steps that do not fit whole are done one row at a time after the loop.
+// The factored rows settle into a constant run (chConstantRuns); the steps
+// inside it run as a loop of their own with the run's values as uniform
+// constants and no ring, and divide in two operations when the host has
+// certified the run's pivot (chDivCertified).
 // ---------------------------------------------------------------------------
 constexpr int SR = 16;  // rows per step (settleStep takes 16)
 
