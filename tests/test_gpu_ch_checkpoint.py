@@ -98,3 +98,47 @@ def test_server_resume_matches_uninterrupted(gpu, tmp_path):
     assert srv.query(other)[0] == 2
     srv.close()
     assert bits_equal(got, c10), mismatches(got, c10)
+
+
+def test_field_levels_every_residue(gpu, tmp_path):
+    """The solver keeps its time levels in three buffers used in turn, and a
+    run of k steps fuses each step's final phase with the next step's rhs
+    except the last.  Runs of 1 to 5 steps in a row reach every buffer
+    residue and every run shape; after each run both levels and a
+    checkpoint written then equal the oracle."""
+    n = 96
+    c0 = ic(n, 11)
+    s = api.CHSolver(api.ch_params(n), c0)
+    rc, rco = c0.ravel().copy(), c0.ravel().copy()
+    ref = oracle.CH(n)
+    done = 0
+    for k in (1, 2, 3, 1, 4, 5, 2):
+        s.run(k)
+        ref.step(rc, rco, steps=k)
+        done += k
+        c, co = s.field()
+        assert mismatches(c, rc.reshape(n, n)) == 0, (done, k)
+        assert mismatches(co, rco.reshape(n, n)) == 0, (done, k)
+        path = str(tmp_path / f"ch{done}.ckpt")
+        s.checkpoint(path)
+        r = api.CHSolver.restore(path)
+        cr, cor = r.field()
+        r.close()
+        assert bits_equal(cr, c) and bits_equal(cor, co), done
+    s.close()
+
+
+def test_server_short_requests(gpu):
+    """Requests of 1 to 4 steps share a lane, so its batches are of every
+    length from 1 (a lone rhs and final) up; each result equals the oracle."""
+    sizes = [48, 48, 48, 48, 32]
+    steps = [1, 2, 3, 4, 1]
+    ics = [ic(n, 60 + r) for r, n in enumerate(sizes)]
+    srv = api.CHServer(budget_bytes=8 * api.ch_footprint(48), max_slots=8)
+    ids = [srv.submit(api.ch_params(n), k, c) for n, k, c in zip(sizes, steps, ics)]
+    srv.run_until_idle()
+    for r, (n, k, c) in enumerate(zip(sizes, steps, ics)):
+        rc, rco = c.ravel().copy(), c.ravel().copy()
+        oracle.CH(n).step(rc, rco, steps=k)
+        assert mismatches(srv.result(ids[r]), rc.reshape(n, n)) == 0, (n, k)
+    srv.close()
