@@ -3,7 +3,7 @@
 //
 // float: for each divisor every significand of x in [1, 2) is tried (2^23
 // values), so "the division is correctly rounded for every x" is known
-// exactly; the check's verdict must agree with it in both directions (a
+// exactly; the check's answer must agree with it in both directions (a
 // certified divisor with a failing x would make the fast path wrong; a
 // refused divisor without one would mean the list of candidates is not the
 // failing set).

@@ -2,7 +2,7 @@
 column solve inside the factors' constant run (chDivCertified,
 custen_amd/csrc/ch_setup.h; the kernel divides as fma(x, zh, RN(x zl)) only
 for a certified pivot).  In single precision every numerator significand is
-tried, so the check's verdict is compared with the truth in both
+tried, so the check's answer is compared with the truth in both
 directions; in double precision certified divisors (the CH pivots at eight
 grid sizes and random ones) are tried on random numerators of every allowed
 exponent.  The GPU parity tests check the kernel against the oracle, which
